@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark: batched uneven 2-set intersection on one MI355X.
+
+Workload (the batched configuration of this path): 65,536 pairs of sorted unique u32; larger
+size log-uniform in [2^6, 2^14]; smaller = larger / r, r log-uniform in [1, 1024]; element space
+10 x larger; selectivity ~ U[0,1] with shared elements planted first.  One step = one batch
+intersection (plan + one persistent launch) producing per-pair counts, per-pair offsets and the
+compacted results, with the inputs already resident in HBM.
+
+Printed (one JSON line): whole-job throughput in input elements/s (E = sum |A|+|B|, the
+reference's element_count convention, scripts/results/process.py:62-77), the main kernel's
+roofline figures (HIP events on the launch stream), the plain one-launch-per-pair GPU path on a
+sample of pairs, and the CPU baselines (the reference's own QFilter and the branchless_merge
+restatement, one pair per host thread).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def algorithmic_bytes(E, out_total, P):
+    # inputs once (4E), offsets read (2 x 8 B per pair), outputs once (4 per element),
+    # per-pair count (4 B) and offset (8 B) written
+    return 4 * E + 16 * P + 4 * out_total + 12 * P
+
+
+def cpu_baselines(A, a_off, B, b_off, E, budget_s):
+    """Times the reference's QFilter (compiled from its source) and the branchless_merge
+    restatement over the whole batch, one pair per host thread; repeats until ~budget_s."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    res = {}
+    if oracle.ref_available():
+        rb = oracle.RefQfilterBatch(A, a_off, B, b_off)
+        rb.run(threads)  # warm
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            rb.run(threads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 2 or reps >= 200:
+                break
+        res["reference_qfilter_c"] = {"gelems": E * reps / el / 1e9, "reps": reps, "seconds": el}
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.batch_intersect("branchless_merge", A, a_off, B, b_off, nthreads=threads, with_output=True)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s / 2 or reps >= 200:
+            break
+    res["port_branchless_merge"] = {"gelems": E * reps / el / 1e9, "reps": reps, "seconds": el}
+    return res, threads
+
+
+def plain_gpu(torch, A, a_off, B, b_off, sample, stream):
+    """The plain drop-in: one launch sequence per pair (device-resident inputs), back to back on
+    one stream, over an evenly spaced sample of pairs."""
+    import ctypes
+
+    from simd_set_operations_amd import _native
+    from simd_set_operations_amd.intersect import algo_id
+
+    lib = _native.hip()
+    algo = algo_id("hip_branchless_merge")
+    P = a_off.size - 1
+    idx = np.linspace(0, P - 1, num=min(sample, P)).astype(np.int64)
+    # each pair 16-byte aligned in its own pool slot
+    sets_a = [A[a_off[p]:a_off[p + 1]] for p in idx]
+    sets_b = [B[b_off[p]:b_off[p + 1]] for p in idx]
+
+    def pack(sets):
+        lens = np.array([s.size for s in sets], np.int64)
+        pad = (lens + 3) // 4 * 4
+        start = np.concatenate([[0], np.cumsum(pad)[:-1]])
+        buf = np.zeros(int(pad.sum()) + 4, np.uint32)
+        for s, st in zip(sets, start):
+            buf[st:st + s.size] = s
+        return torch.from_numpy(buf.view(np.int32)).cuda(), start, lens
+
+    dA, sa, la = pack(sets_a)
+    dB, sb, lb = pack(sets_b)
+    cap = int(np.minimum(la, lb).max()) + 4
+    dO = torch.empty(cap, dtype=torch.int32, device="cuda")
+    dT = torch.zeros(2, dtype=torch.int64, device="cuda")
+    pa, pb, po, pt = dA.data_ptr(), dB.data_ptr(), dO.data_ptr(), dT.data_ptr()
+    s = stream.cuda_stream
+
+    def one(k):
+        _native.check(lib.setops_hip_intersect2_dev(algo, pa + 4 * int(sa[k]), None, int(la[k]), pb + 4 * int(sb[k]),
+                                                    None, int(lb[k]), po, None, cap, pt, s))
+
+    for k in range(min(16, len(idx))):
+        one(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(len(idx)):
+        one(k)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    E_s = int(la.sum() + lb.sum())
+    return {"gelems": E_s / el / 1e9, "pairs": int(len(idx)), "us_per_pair": el / len(idx) * 1e6}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--npairs", type=int, default=65536)
+    ap.add_argument("--cpu-budget", type=float, default=6.0, help="seconds of CPU baseline timing")
+    ap.add_argument("--plain-sample", type=int, default=2048)
+    ap.add_argument("--no-baselines", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from simd_set_operations_amd import datagen, intersect
+    from simd_set_operations_amd.batch import DeviceBatch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    intersect.init(0)
+
+    t0 = time.perf_counter()
+    A, a_off, B, b_off, shared = datagen.uneven_batch(npairs=args.npairs, with_shared=True)
+    P = a_off.size - 1
+    E = int(A.size + B.size)
+    log(f"generated P={P} E={E} in {time.perf_counter() - t0:.1f}s")
+
+    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        db.run(stream)
+    torch.cuda.synchronize()
+    # correctness gate on the bench workload: per-pair counts equal the planted shared counts
+    off, cnt, _ = db.results()
+    if not np.array_equal(cnt, shared):
+        log("per-pair counts differ from the planted shared counts")
+        sys.exit(3)
+    out_total = int(off[-1])
+    log(f"warmup ok: {out_total} results, info={db.info()}")
+
+    db.kernel_time()  # reset
+    db.set_timing(True)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        db.run(stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    step_ms = ev0.elapsed_time(ev1) / args.steps
+    kern_ms_total, runs = db.kernel_time()
+    kern_ms = kern_ms_total / max(runs, 1)
+    db.set_timing(False)
+    value = E / (step_ms / 1e3) / 1e9
+    Bytes = algorithmic_bytes(E, out_total, P)
+    achieved = Bytes / (kern_ms / 1e3) / 1e9
+    log(f"step {step_ms:.4f} ms, main kernel {kern_ms:.4f} ms, {value:.1f} Gelem/s, {achieved:.0f} GB/s")
+
+    extra = {}
+    # Counter-visitor mode (count-only): same tiles, no output placement
+    for _ in range(2):
+        db.run(stream, count_only=True)
+    torch.cuda.synchronize()
+    db.kernel_time()
+    db.set_timing(True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        db.run(stream, count_only=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    co_ms = ev0.elapsed_time(ev1) / args.steps
+    co_k, co_runs = db.kernel_time()
+    db.set_timing(False)
+    words, _ = db.totals()
+    if words != out_total:
+        log(f"count-only total {words} differs from {out_total}")
+        sys.exit(3)
+    extra["count_only"] = {"ms_per_step": co_ms, "kernel_ms": co_k / max(co_runs, 1), "gelems": E / (co_ms / 1e3) / 1e9}
+    log(f"count-only: {extra['count_only']}")
+    cpu = None
+    if not args.no_baselines:
+        pg = plain_gpu(torch, A, a_off, B, b_off, args.plain_sample, stream)
+        extra["plain_gpu_per_pair"] = pg
+        extra["speedup_vs_plain_gpu"] = value / pg["gelems"]
+        log(f"plain per-pair GPU: {pg}")
+        res, threads = cpu_baselines(A, a_off, B, b_off, E, args.cpu_budget)
+        best_kind, best = max(res.items(), key=lambda kv: kv[1]["gelems"])
+        cpu = {
+            "value": best["gelems"],
+            "unit": "Gelem/s",
+            "cores": threads,
+            "kind": "reference" if best_kind.startswith("reference") else "port",
+            "sample": f"full {P}-pair batch x {best['reps']} repetitions, one pair per host thread "
+                      f"({best_kind})",
+        }
+        extra["cpu_all"] = res
+        extra["speedup_vs_cpu"] = value / best["gelems"]
+        log(f"cpu: {res}")
+
+    rec = {
+        "metric": "batched intersect Gelem/s + %HBM roofline; p50/p99 query latency vs plain",
+        "value": value,
+        "unit": "Gelem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded SplitMix64 generator, planted shared elements)",
+        "config": {
+            "workload": "uneven_batch",
+            "pairs": P,
+            "elements": E,
+            "results": out_total,
+            "larger_log2_range": [6, 14],
+            "max_skew": 1024,
+            "density": 0.1,
+            "algorithm": "hip_branchless_merge",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "merge_tile_kernel",
+            "kernel_ms": kern_ms,
+            "algorithmic_bytes": Bytes,
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/*
+ * setops_hip.h -- C ABI of the MI355X (gfx950) sorted-u32 intersection library.
+ *
+ * This is the drop-in boundary for the sorted-array intersection path of
+ * UNSW-database/simd_set_operations.  Every entry point names the reference interface it
+ * replaces (paths relative to that repository).  Conventions shared by all entry points:
+ *
+ *   - Sets are sorted, strictly increasing uint32_t arrays.  (The reference harness passes i32,
+ *     benchmark/src/datafile.rs:25; for values below 2^31 the orders agree.)
+ *   - Results are the intersection in ascending order, identical element for element to the
+ *     reference's branchless_merge (setops/src/intersect/merge.rs:39-60) and svs_generic
+ *     (setops/src/intersect/svs.rs:50-73).
+ *   - Return value: SETOPS_OK (0) or a negative SETOPS_ERR_* code; setops_hip_last_error()
+ *     gives a message for the calling thread.  The reference has no error channel
+ *     (setops/ffi/qfilter/qfilter.h:8-39 return a bare count); misuse that the reference leaves
+ *     undefined (short output buffer, unsorted sizes) is reported here instead.
+ *   - Output capacity follows the reference harness: min(|A|,|B|) for a 2-set call
+ *     (benchmark/src/timer/harness.rs:89), the smallest set for a k-set call.
+ *   - "Host" entry points take host pointers and are synchronous, one GPU launch sequence per
+ *     call -- the plain per-pair drop-in.  "Batch" and "_dev" entry points take device pointers,
+ *     run on the given HIP stream (NULL = the HIP null stream) and return without synchronising.
+ */
+#ifndef SETOPS_HIP_H
+#define SETOPS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SETOPS_OK 0
+#define SETOPS_ERR_INVALID (-1)      /* null pointer, bad size, misaligned pool, bad algorithm use */
+#define SETOPS_ERR_CAPACITY (-2)     /* output capacity too small for the result */
+#define SETOPS_ERR_UNKNOWN_ALGO (-3) /* algorithm id / name not registered */
+#define SETOPS_ERR_HIP (-4)          /* HIP runtime error (message names the call) */
+#define SETOPS_ERR_NOMEM (-5)        /* device allocation failed or budget exceeded */
+#define SETOPS_ERR_UNSUPPORTED (-6)  /* operation not available for this algorithm */
+#define SETOPS_ERR_NO_DEVICE (-7)    /* no usable gfx950 device */
+
+/* ---- library state ----------------------------------------------------------------------- */
+
+/* Selects and initialises the device (idempotent).  Every other call initialises device 0 on
+ * first use if this was not called.  No reference counterpart (the reference is CPU-only). */
+int setops_hip_init(int device);
+
+/* Message for the last error on the calling thread ("" if none). */
+const char *setops_hip_last_error(void);
+
+/* Library version string. */
+const char *setops_hip_version(void);
+
+/* ---- algorithm registry (benchmark/src/timer.rs:75-276: name -> function table) ---------- */
+
+#define SETOPS_ALGO_TWOSET 1u  /* plain 2-set entry (timer.rs:75-169 try_parse_twoset) */
+#define SETOPS_ALGO_BSR 2u     /* BSR 2-set entry (timer.rs:211-259 try_parse_bsr) */
+#define SETOPS_ALGO_KSET 4u    /* usable for k sets through svs_generic (timer.rs:190-198) */
+
+/* Number of registered entries; ids are 0 .. count-1. */
+int setops_hip_algo_count(void);
+/* Id of a registered name (e.g. "hip_branchless_merge"), or SETOPS_ERR_UNKNOWN_ALGO. */
+int setops_hip_algo_lookup(const char *name);
+/* Name of an id, or NULL. */
+const char *setops_hip_algo_name(int algo);
+/* SETOPS_ALGO_* flags of an id, or 0. */
+unsigned setops_hip_algo_flags(int algo);
+
+/* ---- plain 2-set, host pointers --------------------------------------------------------
+ * Replaces Intersect2C (setops/src/intersect.rs:41) as bound for qfilter_c
+ * (setops/src/intersect/qfilter_c.rs:6-22 -> setops/ffi/qfilter/qfilter.cpp:221
+ * intersect_qfilter_uint_b4_v2).  Writes A∩B ascending to out[0 .. *out_len).  out may be NULL
+ * for a count-only call (the Counter visitor, setops/src/visitor.rs:24-48); otherwise out_cap
+ * must be >= the result size (min(na, nb) always suffices). */
+int setops_hip_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out,
+                          size_t out_cap, size_t *out_len);
+
+/* ---- 2-set (plain or BSR), device pointers, asynchronous --------------------------------
+ * The plain one-launch-per-pair drop-in for device-resident sets: the same contract as
+ * setops_hip_intersect2 / setops_hip_intersect2_bsr, but the inputs (16-byte aligned) and outputs
+ * are device pointers and the call returns after enqueueing on `stream` (NULL = null stream).
+ * d_totals[0] receives the result length (words for BSR) and d_totals[1] the element count.
+ * d_a_states / d_b_states / d_out_states are NULL for plain entries.  d_out (and d_out_states)
+ * NULL selects count-only.  Calls share one workspace: keep them on one stream. */
+int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na, const uint32_t *d_b,
+                              const uint32_t *d_b_states, size_t nb, uint32_t *d_out, uint32_t *d_out_states,
+                              size_t out_cap, unsigned long long *d_totals, void *stream);
+
+/* ---- BSR 2-set, host pointers ------------------------------------------------------------
+ * Replaces Intersect2Bsr (setops/src/bsr.rs:16) / UnsafeIntersectBsr
+ * (benchmark/src/timer/harness.rs:17) and intersect_qfilter_bsr_b4_v2 (qfilter.h:24-27).
+ * Inputs are BsrVec layouts (bsr.rs:48-75): bases strictly increasing, states non-zero.  Output is
+ * the (base, state_a & state_b) words with non-zero state, bases ascending; *out_words is their
+ * number and *out_elems (may be NULL) the popcount total (Counter::visit_bsr, visitor.rs:399-403).
+ * bases_out/states_out may both be NULL for a count-only call. */
+int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t *states_a, size_t na,
+                              const uint32_t *bases_b, const uint32_t *states_b, size_t nb, uint32_t *bases_out,
+                              uint32_t *states_out, size_t out_cap, size_t *out_words, uint64_t *out_elems);
+
+/* ---- k-set, host pointers ------------------------------------------------------------------
+ * Replaces svs_generic (setops/src/intersect/svs.rs:50-73) driven by a 2-set entry, as timed by
+ * time_svs (benchmark/src/timer/harness.rs:152-172): sets[0] ∩ sets[1], then the running result
+ * ∩ sets[2], ... in the order given (the caller orders smallest-first).  k >= 2. */
+int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, size_t k, uint32_t *out,
+                   size_t out_cap, size_t *out_len);
+
+/* ---- device-resident batches ----------------------------------------------------------------
+ * A batch is P pairs in CSR form: pair p is (A[a_off[p] .. a_off[p+1]), B[b_off[p] .. b_off[p+1])),
+ * with a_off[0] == b_off[0] == 0, a_off[P] == a_len, b_off[P] == b_len.  A, B (and the BSR
+ * state arrays) must be 16-byte aligned device pointers.  There is no single reference
+ * counterpart: this is the batched form of Intersect2 / Intersect2Bsr that the reference's
+ * benchmark loop calls once per datafile pair (benchmark/src/bin/benchmark.rs:180-243).
+ *
+ * Plain batches pass NULL for a_states/b_states; BSR batches pass the state arrays and a BSR
+ * algorithm.  Planning depends only on the sizes, so one handle can be run many times. */
+typedef struct setops_hip_batch setops_hip_batch;
+
+int setops_hip_batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const uint32_t *d_a_states,
+                            uint64_t a_len, const uint64_t *d_a_off, const uint32_t *d_b, const uint32_t *d_b_states,
+                            uint64_t b_len, const uint64_t *d_b_off, uint64_t npairs);
+
+/* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
+ *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
+ *                          both NULL selects count-only mode (out_cap ignored)
+ *   d_out_off            : [P+1] u64, pair p's results start at d_out_off[p]; d_out_off[P] = total
+ *   d_counts             : [P] u32 result count of each pair (words for BSR)
+ *   d_totals             : [2] u64 totals (words, elements) -- elements = popcount sum for BSR
+ * Any output pointer may be NULL.  If the results exceed out_cap the writes are clipped and
+ * setops_hip_batch_check reports SETOPS_ERR_CAPACITY. */
+int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_out_states, uint64_t out_cap,
+                         uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals, void *stream);
+
+/* Synchronises the handle's last run and reports SETOPS_ERR_CAPACITY if its output was clipped. */
+int setops_hip_batch_check(setops_hip_batch *handle);
+
+/* Enables per-run HIP event timing of the main intersection kernel (0 = off). */
+int setops_hip_batch_set_timing(setops_hip_batch *handle, int enabled);
+
+/* Synchronises and returns the summed main-kernel time (ms) and number of timed runs since the
+ * last call; resets the accumulator. */
+int setops_hip_batch_kernel_time(setops_hip_batch *handle, double *total_ms, uint64_t *runs);
+
+/* Number of tiles and the persistent grid size of the handle (for reporting). */
+int setops_hip_batch_info(setops_hip_batch *handle, uint64_t *ntiles, uint64_t *grid, uint64_t *units);
+
+int setops_hip_batch_destroy(setops_hip_batch *handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SETOPS_HIP_H */

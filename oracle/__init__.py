@@ -1,0 +1,300 @@
+"""The oracle: CPU restatement of the reference's intersection path (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, and
+only as the checker (or the timed CPU baseline) -- never as the thing measured or shipped.
+
+* ``liboracle.so``            -- setops_oracle.c, one C function per reference function (each
+  cites the reference file:line it restates).
+* ``_ref/libqfilter_ref.so``  -- the reference's own setops/ffi/qfilter/qfilter.cpp, compiled
+  from its source by oracle/Makefile (absent when the reference checkout was unavailable at
+  build time).
+
+Pinning (what checks each oracle function, see tests/test_oracle.py):
+  branchless_merge / naive_merge / galloping / binary_search_intersect / shuffling / qfilter:
+      the reference's known answers (setops/tests/unit_tests.rs:6-59), numpy.intersect1d (a
+      library routine), brute-force membership on tiny universes, the invariant trio of
+      setops/tests/testlib/properties.rs:2-49, the compiled reference qfilter (_ref), and the
+      committed golden vectors written from _ref.
+  branchless_merge_bsr / galloping_bsr / shuffling_bsr / qfilter_bsr:
+      decode(result) == plain intersection, the compiled reference intersect_qfilter_bsr_b4_v2.
+  bsr_from_sorted / bsr_to_sorted: round trip, popcount == length, bases strictly increasing.
+  svs: left fold of numpy.intersect1d == k-way intersection (reduce over all sets).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(_HERE, "liboracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libqfilter_ref.so")
+
+NAIVE_MERGE = 0
+BRANCHLESS_MERGE = 1
+GALLOPING = 2
+BINARY_SEARCH = 3
+SHUFFLING4 = 4
+SHUFFLING8 = 5
+SHUFFLING16 = 6
+QFILTER = 7
+
+ALGOS = {
+    "naive_merge": NAIVE_MERGE,
+    "branchless_merge": BRANCHLESS_MERGE,
+    "galloping": GALLOPING,
+    "binary_search": BINARY_SEARCH,
+    "shuffling_sse": SHUFFLING4,
+    "shuffling_avx2": SHUFFLING8,
+    "shuffling_avx512": SHUFFLING16,
+    "qfilter": QFILTER,
+}
+
+_c = ctypes
+_vp = _c.c_void_p
+_sz = _c.c_size_t
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise OSError(f"oracle not built: {ORACLE_LIB} (run `make -C oracle`)")
+        L = _c.CDLL(ORACLE_LIB)
+        for n in ("naive_merge", "branchless_merge", "galloping", "binary_search_intersect"):
+            f = getattr(L, "oracle_" + n)
+            f.restype = _sz
+            f.argtypes = [_vp, _sz, _vp, _sz, _vp]
+        L.oracle_shuffling.restype = _sz
+        L.oracle_shuffling.argtypes = [_c.c_uint, _vp, _sz, _vp, _sz, _vp]
+        L.oracle_qfilter.restype = _sz
+        L.oracle_qfilter.argtypes = [_vp, _sz, _vp, _sz, _vp]
+        L.oracle_qfilter_classify.restype = _c.c_int
+        L.oracle_qfilter_classify.argtypes = [_c.c_uint]
+        for n in ("branchless_merge_bsr", "galloping_bsr", "qfilter_bsr"):
+            f = getattr(L, "oracle_" + n)
+            f.restype = _sz
+            f.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
+        L.oracle_shuffling_bsr.restype = _sz
+        L.oracle_shuffling_bsr.argtypes = [_c.c_uint, _vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
+        L.oracle_bsr_from_sorted.restype = _sz
+        L.oracle_bsr_from_sorted.argtypes = [_vp, _sz, _vp, _vp]
+        L.oracle_bsr_to_sorted.restype = _sz
+        L.oracle_bsr_to_sorted.argtypes = [_vp, _vp, _sz, _vp]
+        L.oracle_bsr_count.restype = _c.c_uint64
+        L.oracle_bsr_count.argtypes = [_vp, _sz]
+        L.oracle_binary_search.restype = _sz
+        L.oracle_binary_search.argtypes = [_vp, _c.c_uint32, _c.c_int64, _c.c_int64]
+        L.oracle_intersect2.restype = _sz
+        L.oracle_intersect2.argtypes = [_c.c_int, _vp, _sz, _vp, _sz, _vp]
+        L.oracle_svs.restype = _sz
+        L.oracle_svs.argtypes = [_c.c_int, _vp, _vp, _sz, _vp, _vp, _c.POINTER(_vp)]
+        L.oracle_batch_intersect.restype = _c.c_int
+        L.oracle_batch_intersect.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """The reference's compiled qfilter.cpp (raises if it was never built)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise OSError(f"reference build absent: {REF_LIB}")
+        R = _c.CDLL(REF_LIB)
+        for n in ("intersect_qfilter_uint_b4", "intersect_qfilter_uint_b4_v2"):
+            f = getattr(R, n)
+            f.restype = _c.c_int
+            f.argtypes = [_vp, _c.c_int, _vp, _c.c_int, _vp]
+        for n in ("intersect_qfilter_bsr_b4", "intersect_qfilter_bsr_b4_v2"):
+            f = getattr(R, n)
+            f.restype = _c.c_int
+            f.argtypes = [_vp, _vp, _c.c_int, _vp, _vp, _c.c_int, _vp, _vp]
+        _ref = R
+    return _ref
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None and a.size else None
+
+
+def intersect2(algo, a, b):
+    """Run one oracle 2-set function (name or id); returns the visited values."""
+    code = ALGOS[algo] if isinstance(algo, str) else algo
+    a, b = _u32(a), _u32(b)
+    out = np.zeros(max(min(a.size, b.size), 1), np.uint32)
+    n = lib().oracle_intersect2(code, _p(a), a.size, _p(b), b.size, _p(out))
+    return out[:n].copy()
+
+
+def count2(algo, a, b):
+    code = ALGOS[algo] if isinstance(algo, str) else algo
+    a, b = _u32(a), _u32(b)
+    return int(lib().oracle_intersect2(code, _p(a), a.size, _p(b), b.size, None))
+
+
+def branchless_merge(a, b):
+    return intersect2(BRANCHLESS_MERGE, a, b)
+
+
+def bsr_from_sorted(v):
+    v = _u32(v)
+    bases = np.zeros(max(v.size, 1), np.uint32)
+    states = np.zeros(max(v.size, 1), np.uint32)
+    n = lib().oracle_bsr_from_sorted(_p(v), v.size, _p(bases), _p(states))
+    return bases[:n].copy(), states[:n].copy()
+
+
+def bsr_to_sorted(bases, states):
+    bases, states = _u32(bases), _u32(states)
+    out = np.zeros(max(32 * bases.size, 1), np.uint32)
+    n = lib().oracle_bsr_to_sorted(_p(bases), _p(states), bases.size, _p(out))
+    return out[:n].copy()
+
+
+def bsr_intersect(kind, ba, sa, bb, sb, width=4):
+    """kind in {'branchless_merge_bsr', 'galloping_bsr', 'shuffling_bsr', 'qfilter_bsr'}."""
+    ba, sa, bb, sb = _u32(ba), _u32(sa), _u32(bb), _u32(sb)
+    cap = max(min(ba.size, bb.size), 1)
+    ob = np.zeros(cap, np.uint32)
+    os_ = np.zeros(cap, np.uint32)
+    L = lib()
+    if kind == "shuffling_bsr":
+        n = L.oracle_shuffling_bsr(width, _p(ba), _p(sa), ba.size, _p(bb), _p(sb), bb.size, _p(ob), _p(os_))
+    else:
+        n = getattr(L, "oracle_" + kind)(_p(ba), _p(sa), ba.size, _p(bb), _p(sb), bb.size, _p(ob), _p(os_))
+    return ob[:n].copy(), os_[:n].copy()
+
+
+def svs(algo, sets):
+    """svs_generic fold with an oracle 2-set function; returns the final intersection."""
+    code = ALGOS[algo] if isinstance(algo, str) else algo
+    arrs = [_u32(s) for s in sets]
+    k = len(arrs)
+    cap = max(max(a.size for a in arrs), 1)
+    left = np.zeros(cap, np.uint32)
+    right = np.zeros(cap, np.uint32)
+    ptrs = (_vp * k)(*[a.ctypes.data if a.size else None for a in arrs])
+    lens = (_sz * k)(*[a.size for a in arrs])
+    res = _vp()
+    n = lib().oracle_svs(code, _c.cast(ptrs, _vp), _c.cast(lens, _vp), k, _p(left), _p(right), _c.byref(res))
+    which = left if res.value == left.ctypes.data else right
+    return which[:n].copy()
+
+
+def batch_intersect(algo, A, a_off, B, b_off, nthreads=1, with_output=True):
+    """One intersection per host thread over a CSR batch.  Returns (counts, cap_off, strided out)."""
+    code = ALGOS[algo] if isinstance(algo, str) else algo
+    A, B = _u32(A), _u32(B)
+    a_off = np.ascontiguousarray(a_off, np.uint64)
+    b_off = np.ascontiguousarray(b_off, np.uint64)
+    P = a_off.size - 1
+    caps = np.minimum(np.diff(a_off), np.diff(b_off))
+    cap_off = np.zeros(P + 1, np.uint64)
+    np.cumsum(caps, out=cap_off[1:])
+    out = np.zeros(max(int(cap_off[-1]), 1), np.uint32) if with_output else None
+    counts = np.zeros(P, np.uint32)
+    rc = lib().oracle_batch_intersect(code, nthreads, _p(A), _p(a_off), _p(B), _p(b_off), P, _p(cap_off),
+                                      _p(out) if with_output else None, _p(counts))
+    if rc != 0:
+        raise RuntimeError("oracle batch failed")
+    return counts, cap_off, out
+
+
+def compact(counts, cap_off, strided):
+    """Capacity-strided per-pair outputs -> (offsets, concatenated data)."""
+    P = counts.size
+    off = np.zeros(P + 1, np.uint64)
+    np.cumsum(counts.astype(np.uint64), out=off[1:])
+    idx = np.concatenate([np.arange(int(cap_off[p]), int(cap_off[p]) + int(counts[p])) for p in range(P)]) \
+        if P and off[-1] else np.zeros(0, np.int64)
+    return off, strided[idx] if idx.size else np.zeros(0, np.uint32)
+
+
+def ref_qfilter(a, b, v2=True):
+    """The reference's intersect_qfilter_uint_b4(_v2) on 16-byte aligned, padded copies
+    (it reads up to 16 bytes past each input and writes up to 12 bytes past the count)."""
+    R = ref()
+    a, b = _u32(a), _u32(b)
+    if a.size and (a.max() >= 1 << 31) or b.size and (b.max() >= 1 << 31):
+        raise ValueError("the reference compares signed int: values must be < 2^31")
+    A = _aligned(a, a.size + 8)
+    B = _aligned(b, b.size + 8)
+    out = _aligned(np.zeros(0, np.uint32), min(a.size, b.size) + 8)
+    f = R.intersect_qfilter_uint_b4_v2 if v2 else R.intersect_qfilter_uint_b4
+    n = f(_p(A), a.size, _p(B), b.size, _p(out))
+    return out[:n].copy()
+
+
+def ref_qfilter_bsr(ba, sa, bb, sb, v2=True):
+    R = ref()
+    cap = min(len(ba), len(bb)) + 8
+    A, As = _aligned(ba, len(ba) + 8), _aligned(sa, len(sa) + 8)
+    B, Bs = _aligned(bb, len(bb) + 8), _aligned(sb, len(sb) + 8)
+    ob, os_ = _aligned(np.zeros(0, np.uint32), cap), _aligned(np.zeros(0, np.uint32), cap)
+    f = R.intersect_qfilter_bsr_b4_v2 if v2 else R.intersect_qfilter_bsr_b4
+    n = f(_p(A), _p(As), len(ba), _p(B), _p(Bs), len(bb), _p(ob), _p(os_))
+    return ob[:n].copy(), os_[:n].copy()
+
+
+def _aligned(src, n):
+    """uint32 array of length n (zero padded) whose data pointer is 64-byte aligned."""
+    raw = np.zeros(n + 16, np.uint32)
+    off = (-raw.ctypes.data % 64) // 4
+    arr = raw[off:off + n]
+    src = np.asarray(src, np.uint32)
+    arr[: src.size] = src
+    return arr
+
+
+class RefQfilterBatch:
+    """The reference's intersect_qfilter_uint_b4_v2 over a whole CSR batch, one pair per host
+    thread, on a padded copy that meets its alignment/overrun contract (built once, outside any
+    timed region).  Values must be < 2^31 (the reference compares signed int)."""
+
+    def __init__(self, A, a_off, B, b_off):
+        R = ref()
+        R.ref_batch_qfilter.restype = _c.c_int
+        R.ref_batch_qfilter.argtypes = [_c.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp]
+        a_off = np.asarray(a_off, np.uint64)
+        b_off = np.asarray(b_off, np.uint64)
+        self.P = a_off.size - 1
+        self.a_len = np.diff(a_off).astype(np.uint32)
+        self.b_len = np.diff(b_off).astype(np.uint32)
+        self.A, self.a_start = self._pad(np.asarray(A, np.uint32), a_off, self.a_len)
+        self.B, self.b_start = self._pad(np.asarray(B, np.uint32), b_off, self.b_len)
+        caps = np.minimum(self.a_len, self.b_len).astype(np.uint64) + 4
+        caps = (caps + 3) // 4 * 4
+        self.out_start = np.zeros(self.P, np.uint64)
+        np.cumsum(caps[:-1], out=self.out_start[1:])
+        self.out = _aligned(np.zeros(0, np.uint32), int(caps.sum()) + 16)
+        self.counts = np.zeros(self.P, np.uint32)
+
+    @staticmethod
+    def _pad(pool, off, lens):
+        padded = ((lens.astype(np.uint64) + 4 + 3) // 4) * 4  # >= 16 spare bytes, 16-byte multiple
+        start = np.zeros(lens.size, np.uint64)
+        np.cumsum(padded[:-1], out=start[1:])
+        buf = _aligned(np.zeros(0, np.uint32), int(padded.sum()) + 16)
+        # scatter each set to its padded start (vectorised by repeating starts)
+        rel = np.arange(pool.size, dtype=np.uint64) - np.repeat(off[:-1], lens.astype(np.int64))
+        dst = np.repeat(start, lens.astype(np.int64)) + rel
+        buf[dst.astype(np.int64)] = pool
+        return buf, start
+
+    def run(self, nthreads):
+        rc = ref().ref_batch_qfilter(nthreads, _p(self.A), _p(self.a_start), _p(self.a_len), _p(self.B),
+                                     _p(self.b_start), _p(self.b_len), self.P, _p(self.out), _p(self.out_start),
+                                     _p(self.counts))
+        if rc != 0:
+            raise RuntimeError("reference batch failed")
+        return self.counts

@@ -1,0 +1,150 @@
+"""Device-resident batches of set pairs (the batched form of Intersect2 / Intersect2Bsr).
+
+Device memory comes from PyTorch (plumbing only); the library receives raw pointers through the
+C ABI (``setops_hip_batch_*``).  A :class:`DeviceBatch` uploads a CSR batch once, plans once
+(sizes only), and can be run many times on a stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+_c = ctypes
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise _native.SetopsError(_native.SETOPS_ERR_NO_DEVICE, "no GPU visible to torch")
+    return torch
+
+
+def to_device_u32(arr, torch):
+    """Upload a uint32 numpy array (as int32 bits); returns a tensor whose storage is 16-byte
+    aligned (torch allocations are 256-byte aligned)."""
+    a = np.ascontiguousarray(arr, dtype=np.uint32)
+    t = torch.from_numpy(a.view(np.int32)) if a.size else torch.zeros(4, dtype=torch.int32)
+    return t.to("cuda")
+
+
+def to_device_u64(arr, torch):
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    return torch.from_numpy(a.view(np.int64)).to("cuda")
+
+
+class DeviceBatch:
+    """CSR batch on the GPU: pair p = (A[a_off[p]:a_off[p+1]], B[b_off[p]:b_off[p+1]])."""
+
+    def __init__(self, algo, A, a_off, B, b_off, A_states=None, B_states=None):
+        torch = _torch()
+        from .intersect import algo_id
+
+        lib = _native.hip()
+        self.torch = torch
+        self.algo = algo
+        a_off = np.ascontiguousarray(a_off, dtype=np.uint64)
+        b_off = np.ascontiguousarray(b_off, dtype=np.uint64)
+        if a_off.size != b_off.size or a_off.size < 2:
+            raise ValueError("a_off and b_off must both have P+1 >= 2 entries")
+        if a_off[0] != 0 or b_off[0] != 0:
+            raise ValueError("offsets must start at 0")
+        if np.any(np.diff(a_off.astype(np.int64)) < 0) or np.any(np.diff(b_off.astype(np.int64)) < 0):
+            raise ValueError("offsets must be non-decreasing")
+        self.P = a_off.size - 1
+        self.a_len = int(a_off[-1])
+        self.b_len = int(b_off[-1])
+        if len(A) != self.a_len or len(B) != self.b_len:
+            raise ValueError("pool lengths must equal the last offsets")
+        self.a_sizes = np.diff(a_off)
+        self.b_sizes = np.diff(b_off)
+        self.dA = to_device_u32(A, torch)
+        self.dB = to_device_u32(B, torch)
+        self.dA_off = to_device_u64(a_off, torch)
+        self.dB_off = to_device_u64(b_off, torch)
+        self.bsr = A_states is not None
+        self.dAs = to_device_u32(A_states, torch) if self.bsr else None
+        self.dBs = to_device_u32(B_states, torch) if self.bsr else None
+        h = _c.c_void_p()
+        _native.check(
+            lib.setops_hip_batch_create(
+                _c.byref(h), algo_id(algo), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
+                self.a_len, self.dA_off.data_ptr(), self.dB.data_ptr(), self.dBs.data_ptr() if self.bsr else None,
+                self.b_len, self.dB_off.data_ptr(), self.P,
+            )
+        )
+        self.handle = h
+        self.capacity = int(np.minimum(self.a_sizes, self.b_sizes).sum())
+        cap = max(self.capacity, 4)
+        self.d_out = torch.empty(cap, dtype=torch.int32, device="cuda")
+        self.d_out_s = torch.empty(cap, dtype=torch.int32, device="cuda") if self.bsr else None
+        self.d_out_off = torch.empty(self.P + 1, dtype=torch.int64, device="cuda")
+        self.d_counts = torch.empty(self.P, dtype=torch.int32, device="cuda")
+        self.d_totals = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+    # -- running ----------------------------------------------------------------------------
+    def run(self, stream=None, count_only=False):
+        """Launch one batch intersection on `stream` (torch stream or None = current stream)."""
+        lib = _native.hip()
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        _native.check(
+            lib.setops_hip_batch_run(
+                self.handle,
+                None if count_only else self.d_out.data_ptr(),
+                None if (count_only or not self.bsr) else self.d_out_s.data_ptr(),
+                0 if count_only else self.d_out.numel(),
+                self.d_out_off.data_ptr(),
+                self.d_counts.data_ptr(),
+                self.d_totals.data_ptr(),
+                s.cuda_stream,
+            )
+        )
+
+    def check(self):
+        _native.check(_native.hip().setops_hip_batch_check(self.handle))
+
+    def set_timing(self, on=True):
+        _native.check(_native.hip().setops_hip_batch_set_timing(self.handle, 1 if on else 0))
+
+    def kernel_time(self):
+        """(summed main-kernel ms, runs) since the last call (HIP events on the run stream)."""
+        ms = _c.c_double(0)
+        runs = _c.c_uint64(0)
+        _native.check(_native.hip().setops_hip_batch_kernel_time(self.handle, _c.byref(ms), _c.byref(runs)))
+        return ms.value, runs.value
+
+    def info(self):
+        nt, g, u = _c.c_uint64(), _c.c_uint64(), _c.c_uint64()
+        _native.check(_native.hip().setops_hip_batch_info(self.handle, _c.byref(nt), _c.byref(g), _c.byref(u)))
+        return {"tiles": nt.value, "grid": g.value, "units": u.value}
+
+    # -- results ------------------------------------------------------------------------------
+    def results(self):
+        """Host copies: (out_off[P+1], counts[P], data[, states]) of the last run."""
+        self.torch.cuda.synchronize()
+        self.check()
+        off = self.d_out_off.cpu().numpy().view(np.uint64).copy()
+        cnt = self.d_counts.cpu().numpy().view(np.uint32).copy()
+        total = int(off[-1])
+        data = self.d_out[:total].cpu().numpy().view(np.uint32).copy()
+        if self.bsr:
+            st = self.d_out_s[:total].cpu().numpy().view(np.uint32).copy()
+            return off, cnt, data, st
+        return off, cnt, data
+
+    def totals(self):
+        self.torch.cuda.synchronize()
+        t = self.d_totals.cpu().numpy().view(np.uint64)
+        return int(t[0]), int(t[1])
+
+    def close(self):
+        if self.handle:
+            _native.hip().setops_hip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

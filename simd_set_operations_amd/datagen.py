@@ -1,0 +1,72 @@
+"""Seeded synthetic workloads (wrappers over libsetops_datagen.so, SplitMix64 streams).
+
+The workloads are the benchmark configurations of this path:
+
+* ``plain_pair``   -- one pair of 2^16 + 2^16 sorted unique u32 over {0..2^20-1} with exactly
+  19,661 shared elements (the reference's CPU-runnable 2-set case).
+* ``uneven_batch`` -- 65,536 pairs; larger size log-uniform in [2^6, 2^14], smaller = larger / r
+  with r log-uniform in [1, 1024]; element space = 10 x larger; shared = round(sel x smaller),
+  sel ~ U[0,1].
+
+Both follow the reference generator's planted-shared-then-disjoint semantics
+(benchmark/src/generators.rs:30-110).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+_c = ctypes
+
+PLAIN_PAIR = dict(universe=1 << 20, size=1 << 16, shared=19661, seed=1001)
+UNEVEN = dict(npairs=65536, lmin=6.0, lmax=14.0, log2_skew=10.0, density=0.1, seed=1002)
+
+
+def _p(a):
+    return a.ctypes.data_as(_c.c_void_p)
+
+
+def splitmix64(seed, n):
+    out = np.zeros(n, np.uint64)
+    _native.datagen().setops_gen_splitmix64(seed, _p(out), n)
+    return out
+
+
+def planted_pair(seed, universe, shared, only_a, only_b):
+    a = np.zeros(shared + only_a, np.uint32)
+    b = np.zeros(shared + only_b, np.uint32)
+    rc = _native.datagen().setops_gen_planted_pair(seed, universe, shared, only_a, only_b,
+                                                   _p(a) if a.size else None, _p(b) if b.size else None)
+    if rc != 0:
+        raise ValueError("planted pair does not fit in its element space")
+    return a, b
+
+
+def plain_pair(seed=PLAIN_PAIR["seed"]):
+    """The 1-pair workload: |A| = |B| = 2^16 over {0..2^20-1}, 19,661 shared."""
+    n, s = PLAIN_PAIR["size"], PLAIN_PAIR["shared"]
+    return planted_pair(seed, PLAIN_PAIR["universe"], s, n - s, n - s)
+
+
+def uneven_batch(npairs=UNEVEN["npairs"], seed=UNEVEN["seed"], lmin=UNEVEN["lmin"], lmax=UNEVEN["lmax"],
+                 log2_skew=UNEVEN["log2_skew"], density=UNEVEN["density"], with_shared=False):
+    """Uneven CSR batch: returns (A, a_off, B, b_off[, shared]) with A = the smaller sets."""
+    lib = _native.datagen()
+    a_len = np.zeros(npairs, np.uint64)
+    b_len = np.zeros(npairs, np.uint64)
+    lib.setops_gen_uneven_sizes(seed, npairs, lmin, lmax, log2_skew, density, _p(a_len), _p(b_len))
+    a_off = np.zeros(npairs + 1, np.uint64)
+    b_off = np.zeros(npairs + 1, np.uint64)
+    np.cumsum(a_len, out=a_off[1:])
+    np.cumsum(b_len, out=b_off[1:])
+    A = np.zeros(int(a_off[-1]), np.uint32)
+    B = np.zeros(int(b_off[-1]), np.uint32)
+    shared = np.zeros(npairs, np.uint32)
+    rc = lib.setops_gen_uneven_fill(seed, npairs, lmin, lmax, log2_skew, density, _p(a_off), _p(b_off), _p(A), _p(B),
+                                    _p(shared))
+    if rc != 0:
+        raise RuntimeError("uneven batch generation failed")
+    if with_shared:
+        return A, a_off, B, b_off, shared
+    return A, a_off, B, b_off

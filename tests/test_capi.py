@@ -1,0 +1,90 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol include/setops_hip.h
+declares, the registry answers, and argument errors are reported (not undefined behaviour)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from simd_set_operations_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "setops_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(setops_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "setops_hip_intersect2" in names
+    assert "setops_hip_svs" in names
+    assert "setops_hip_batch_run" in names
+    assert len(names) >= 17
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_native.HIP_LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    assert set(declared_functions()) == set(_native.HIP_SIGNATURES)
+
+
+def test_registry_without_device():
+    lib = _native.hip()
+    n = lib.setops_hip_algo_count()
+    names = [lib.setops_hip_algo_name(i).decode() for i in range(n)]
+    assert "hip_branchless_merge" in names
+    assert "hip_branchless_merge_bsr" in names
+    for i, name in enumerate(names):
+        assert lib.setops_hip_algo_lookup(name.encode()) == i
+        assert name.startswith("hip_")
+    assert lib.setops_hip_algo_flags(lib.setops_hip_algo_lookup(b"hip_branchless_merge")) & _native.ALGO_TWOSET
+    assert lib.setops_hip_algo_flags(lib.setops_hip_algo_lookup(b"hip_branchless_merge_bsr")) & _native.ALGO_BSR
+    assert lib.setops_hip_algo_name(n) is None
+    assert lib.setops_hip_algo_flags(-1) == 0
+
+
+def test_unknown_algorithm_is_an_error():
+    lib = _native.hip()
+    assert lib.setops_hip_algo_lookup(b"branchless_merge_but_wrong") == _native.SETOPS_ERR_UNKNOWN_ALGO
+    assert b"unknown algorithm" in lib.setops_hip_last_error()
+    with pytest.raises(_native.SetopsError) as e:
+        _native.check(lib.setops_hip_algo_lookup(b"nope"))
+    assert e.value.code == _native.SETOPS_ERR_UNKNOWN_ALGO
+
+
+def test_argument_errors_before_device_use():
+    """Errors that the library reports without touching the GPU."""
+    lib = _native.hip()
+    n = ctypes.c_size_t(0)
+    algo = lib.setops_hip_algo_lookup(b"hip_branchless_merge")
+    bsr = lib.setops_hip_algo_lookup(b"hip_branchless_merge_bsr")
+    # NULL input with non-zero length
+    assert lib.setops_hip_intersect2(algo, None, 5, None, 0, None, 0, ctypes.byref(n)) == _native.SETOPS_ERR_INVALID
+    # a BSR entry through the plain call
+    a = np.arange(4, dtype=np.uint32)
+    p = a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.setops_hip_intersect2(bsr, p, 4, p, 4, None, 0, ctypes.byref(n)) == _native.SETOPS_ERR_UNSUPPORTED
+    # unknown id
+    assert lib.setops_hip_intersect2(99, p, 4, p, 4, None, 0, ctypes.byref(n)) == _native.SETOPS_ERR_UNKNOWN_ALGO
+    # svs with fewer than 2 sets (svs.rs:61 asserts)
+    arr = (ctypes.c_void_p * 1)(a.ctypes.data)
+    lens = (ctypes.c_size_t * 1)(4)
+    rc = lib.setops_hip_svs(algo, ctypes.cast(arr, ctypes.c_void_p), ctypes.cast(lens, ctypes.c_void_p), 1, None, 0,
+                            ctypes.byref(n))
+    assert rc == _native.SETOPS_ERR_INVALID
+    # batch handle NULL
+    assert lib.setops_hip_batch_run(None, None, None, 0, None, None, None, None) == _native.SETOPS_ERR_INVALID
+    assert lib.setops_hip_batch_check(None) == _native.SETOPS_ERR_INVALID
+
+
+def test_version_string():
+    assert b"gfx950" in _native.hip().setops_hip_version()

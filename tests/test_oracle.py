@@ -1,0 +1,226 @@
+"""Pins the oracle (CPU, no GPU): every oracle function is checked against something other than
+its own formula -- the reference's known answers, the reference's compiled qfilter.cpp, the
+committed golden vectors, numpy.intersect1d, brute force on tiny universes and the reference's
+invariant trio (setops/tests/testlib/properties.rs:2-49)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
+TWOSET = ["naive_merge", "branchless_merge", "galloping", "binary_search", "shuffling_sse", "shuffling_avx2",
+          "shuffling_avx512", "qfilter"]
+BSR_KINDS = ["branchless_merge_bsr", "galloping_bsr", "shuffling_bsr", "qfilter_bsr"]
+
+
+def golden():
+    return np.load(GOLDEN)
+
+
+def golden_pairs(g):
+    i = 0
+    while f"pair{i}_a" in g:
+        yield i, g[f"pair{i}_a"], g[f"pair{i}_b"]
+        i += 1
+
+
+def sorted_set(rng, n, universe):
+    return np.unique(rng.integers(0, universe, size=n)).astype(np.uint32)
+
+
+# -- properties.rs:2-49 --------------------------------------------------------------------------
+def prop_intersection_correct(result, sets):
+    result = np.asarray(result, np.int64)
+    increasing = bool(np.all(np.diff(result) > 0))
+    all_common = all(np.isin(result, np.asarray(s, np.int64)).all() for s in sets)
+    common = np.asarray(sets[0], np.int64)
+    for s in sets[1:]:
+        common = common[np.isin(common, np.asarray(s, np.int64))]
+    complete = bool(np.isin(common, result).all())
+    return increasing and all_common and complete
+
+
+@pytest.mark.parametrize("algo", TWOSET)
+def test_known_answers(algo):
+    """setops/tests/unit_tests.rs:6-59 through every oracle 2-set function."""
+    g = golden()
+    for i in range(6):
+        got = oracle.intersect2(algo, g[f"kat{i}_a"], g[f"kat{i}_b"])
+        assert np.array_equal(got, g[f"kat{i}_out"]), (algo, i)
+
+
+def test_const_intersect_answer():
+    # unit_tests.rs:28-33: const_intersect gives [1,3,9,10,15,20]
+    a = [1, 3, 5, 8, 9, 10, 14, 15, 18, 20]
+    b = [1, 2, 3, 4, 9, 10, 11, 15, 20, 21]
+    assert oracle.branchless_merge(a, b).tolist() == [1, 3, 9, 10, 15, 20]
+
+
+@pytest.mark.parametrize("algo", TWOSET)
+def test_golden_vectors(algo):
+    """Outputs of the reference's compiled intersect_qfilter_uint_b4_v2 (tests/golden/)."""
+    g = golden()
+    for i, a, b in golden_pairs(g):
+        # galloping-family functions take (small, large); orientation changes cost, not output
+        got = oracle.intersect2(algo, a, b)
+        assert np.array_equal(got, g[f"pair{i}_out"]), (algo, i)
+
+
+@pytest.mark.parametrize("algo", TWOSET)
+def test_against_numpy(algo):
+    rng = np.random.Generator(np.random.PCG64(7))
+    for trial in range(60):
+        la, lb = rng.integers(0, 3000, size=2)
+        u = int(rng.integers(1, 20000))
+        a, b = sorted_set(rng, la, u), sorted_set(rng, lb, u)
+        exp = np.intersect1d(a, b).astype(np.uint32)
+        assert np.array_equal(oracle.intersect2(algo, a, b), exp), (algo, trial)
+        assert oracle.count2(algo, a, b) == exp.size
+
+
+@pytest.mark.parametrize("algo", TWOSET)
+def test_brute_force_tiny(algo):
+    """Every pair of subsets of a 6-element universe: membership by brute force."""
+    u = 6
+    for ma in range(1 << u):
+        a = np.array([x for x in range(u) if ma >> x & 1], np.uint32)
+        for mb in range(0, 1 << u, 3):
+            b = np.array([x for x in range(u) if mb >> x & 1], np.uint32)
+            exp = [x for x in range(u) if (ma & mb) >> x & 1]
+            assert oracle.intersect2(algo, a, b).tolist() == exp
+
+
+@pytest.mark.parametrize("algo", TWOSET)
+def test_invariant_trio_skewed(algo):
+    # testlib/mod.rs:123-153 SkewedSetPair shape: small 1..128, large 128..8319
+    rng = np.random.Generator(np.random.PCG64(11))
+    for _ in range(40):
+        small = sorted_set(rng, int(rng.integers(1, 128)), 20000)
+        large = np.union1d(sorted_set(rng, int(rng.integers(128, 8319)), 20000), small[: small.size // 2])
+        large = large.astype(np.uint32)
+        r = oracle.intersect2(algo, small, large)
+        assert prop_intersection_correct(r, [small, large])
+
+
+def test_binary_search_semantics():
+    # galloping.rs:110-131: index of the target, else the insertion point lo
+    s = np.array([2, 4, 6, 8], np.uint32)
+    L = oracle.lib()
+    p = s.ctypes.data
+    assert L.oracle_binary_search(p, 6, 0, 3) == 2
+    assert L.oracle_binary_search(p, 5, 0, 3) == 2
+    assert L.oracle_binary_search(p, 1, 0, 3) == 0
+    assert L.oracle_binary_search(p, 9, 0, 3) == 4
+
+
+def test_qfilter_dictionary():
+    # qfilter.rs:254-357 classification: none / single offsets / multi
+    cls = oracle.lib().oracle_qfilter_classify
+    assert cls(0) == -2
+    assert cls(0xFFFF) == -1
+    assert cls(0b0011) == -1  # lane 0 matches b0 and b1
+    # lane 0 -> b2, lane 1 none (points at itself), lane 2 -> b0, lane 3 none
+    m = (1 << 2) | (1 << 8)
+    assert cls(m) == (2 | (1 << 2) | (0 << 4) | (3 << 6))
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference qfilter build absent")
+def test_against_reference_qfilter():
+    """The oracle's qfilter (and every other 2-set function) equals the reference's compiled
+    intersect_qfilter_uint_b4 and _v2 on fresh seeded inputs."""
+    rng = np.random.Generator(np.random.PCG64(99))
+    for _ in range(80):
+        la, lb = rng.integers(0, 5000, size=2)
+        u = int(rng.integers(1, 1 << 16))
+        a, b = sorted_set(rng, la, u), sorted_set(rng, lb, u)
+        r2 = oracle.ref_qfilter(a, b, v2=True)
+        r1 = oracle.ref_qfilter(a, b, v2=False)
+        assert np.array_equal(r1, r2)
+        for algo in TWOSET:
+            assert np.array_equal(oracle.intersect2(algo, a, b), r2)
+
+
+def test_bsr_round_trip():
+    rng = np.random.Generator(np.random.PCG64(3))
+    for _ in range(50):
+        v = sorted_set(rng, int(rng.integers(0, 4000)), int(rng.integers(1, 100000)))
+        b, s = oracle.bsr_from_sorted(v)
+        assert np.all(np.diff(b.astype(np.int64)) > 0)
+        assert np.all(s != 0)
+        assert int(oracle.lib().oracle_bsr_count(s.ctypes.data, s.size)) == v.size
+        assert np.array_equal(oracle.bsr_to_sorted(b, s), v)
+
+
+def test_bsr_known_encoding():
+    # bsr.rs:140-164 by hand: 1, 3 -> base 0 state 0b1010; 32 -> base 1 state 1; 95 -> base 2 bit 31
+    b, s = oracle.bsr_from_sorted([1, 3, 32, 95])
+    assert b.tolist() == [0, 1, 2]
+    assert s.tolist() == [0b1010, 1, 1 << 31]
+
+
+@pytest.mark.parametrize("kind", BSR_KINDS)
+def test_bsr_intersections(kind):
+    """decode(bsr_a ∩ bsr_b) == a ∩ b; word list equals from_sorted(a ∩ b)."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    for _ in range(40):
+        u = int(rng.integers(64, 40000))
+        a, b = sorted_set(rng, int(rng.integers(0, 3000)), u), sorted_set(rng, int(rng.integers(0, 3000)), u)
+        ba, sa = oracle.bsr_from_sorted(a)
+        bb, sb = oracle.bsr_from_sorted(b)
+        for w in (4, 8, 16) if kind == "shuffling_bsr" else (4,):
+            ob, os_ = oracle.bsr_intersect(kind, ba, sa, bb, sb, width=w)
+            exp = np.intersect1d(a, b).astype(np.uint32)
+            eb, es = oracle.bsr_from_sorted(exp)
+            assert np.array_equal(ob, eb) and np.array_equal(os_, es), (kind, w)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference qfilter build absent")
+def test_bsr_against_reference():
+    g = golden()
+    for i, a, b in golden_pairs(g):
+        ba, sa = oracle.bsr_from_sorted(a)
+        bb, sb = oracle.bsr_from_sorted(b)
+        for kind in BSR_KINDS:
+            ob, os_ = oracle.bsr_intersect(kind, ba, sa, bb, sb)
+            assert np.array_equal(ob, g[f"pair{i}_bsr_bases"]), (kind, i)
+            assert np.array_equal(os_, g[f"pair{i}_bsr_states"]), (kind, i)
+
+
+@pytest.mark.parametrize("algo", ["branchless_merge", "galloping", "shuffling_avx2", "qfilter"])
+def test_svs_fold(algo):
+    """svs.rs:50-73: the fold equals the k-way intersection and satisfies the invariant trio
+    (property_tests.rs svs_correct with SetCollection, testlib/mod.rs:168-212)."""
+    rng = np.random.Generator(np.random.PCG64(13))
+    for _ in range(30):
+        k = int(rng.integers(2, 7))
+        common = sorted_set(rng, int(rng.integers(0, 50)), 5000)
+        sets = [np.union1d(sorted_set(rng, int(rng.integers(0, 2000)), 5000), common).astype(np.uint32)
+                for _ in range(k)]
+        sets.sort(key=len)
+        exp = sets[0]
+        for s in sets[1:]:
+            exp = np.intersect1d(exp, s)
+        got = oracle.svs(algo, sets)
+        assert np.array_equal(got, exp.astype(np.uint32))
+        assert prop_intersection_correct(got, sets)
+
+
+def test_batch_runner_matches_pairwise():
+    rng = np.random.Generator(np.random.PCG64(17))
+    P = 200
+    la = rng.integers(0, 300, size=P)
+    lb = rng.integers(0, 300, size=P)
+    As = [sorted_set(rng, int(n), 1000) for n in la]
+    Bs = [sorted_set(rng, int(n), 1000) for n in lb]
+    a_off = np.concatenate([[0], np.cumsum([x.size for x in As])]).astype(np.uint64)
+    b_off = np.concatenate([[0], np.cumsum([x.size for x in Bs])]).astype(np.uint64)
+    A = np.concatenate(As).astype(np.uint32)
+    B = np.concatenate(Bs).astype(np.uint32)
+    counts, cap_off, strided = oracle.batch_intersect("branchless_merge", A, a_off, B, b_off, nthreads=4)
+    off, data = oracle.compact(counts, cap_off, strided)
+    for p in range(P):
+        exp = np.intersect1d(As[p], Bs[p])
+        assert counts[p] == exp.size
+        assert np.array_equal(data[int(off[p]):int(off[p + 1])], exp)
