@@ -81,7 +81,6 @@ struct Context {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     int occ_merge[4] = {0, 0, 0, 0};  // [bsr][count_only] resident blocks per CU
-    uint32_t spin_limit = 64;         // look-back sleeps before a tile goes to the pending area
     std::mutex host_mu;               // serialises the host-pointer entry points
     // grow-only device buffers for the host-pointer entry points
     void *host_buf = nullptr;
@@ -127,7 +126,6 @@ int ensure_init(int device)
     g_ctx.occ_merge[2] = occ;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)merge_kernel_ptr<true, true>(), kNT, 0));
     g_ctx.occ_merge[3] = occ;
-    if (const char *env = std::getenv("SETOPS_SPIN_LIMIT")) g_ctx.spin_limit = (uint32_t)std::strtoul(env, nullptr, 10);
     g_ctx.device = device;
     g_ctx.ready = true;
     return SETOPS_OK;
@@ -159,8 +157,8 @@ struct setops_hip_batch {
     TilePlan plan{};
     uint64_t P = 0;
     int grid_plan = 1;
-    int grid_main[2] = {1, 1};  // [count_only]
     void *ws = nullptr;
+    size_t ws_bytes = 0;
     uint64_t cap_pairs = 0;  // workspace capacity (pairs, tiles)
     uint64_t cap_tiles = 0;
     uint32_t *d_error = nullptr;
@@ -174,19 +172,16 @@ namespace {
 
 int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles);
 
-// Largest persistent grid any main-kernel variant launches (sizes the pending areas).
-uint64_t max_grid()
-{
-    int occ = 1;
-    for (int k = 0; k < 4; k++) occ = std::max(occ, g_ctx.occ_merge[k]);
-    return (uint64_t)occ * (uint64_t)std::max(1, g_ctx.num_cu);
-}
+// Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
+// can produce ((U+1)/2 + 1).
+constexpr uint32_t kStageHalf = kU / 2 + 64;
+constexpr uint32_t kStageWords = 2 * kStageHalf;
 
 // Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields.
 int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
 {
     const uint64_t ntiles = std::max<uint64_t>(1, (utotal + kU - 1) / kU);
-    if (ntiles > 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu tiles", (unsigned long long)ntiles);
+    if (ntiles >= 0x7FFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu tiles", (unsigned long long)ntiles);
     if (P > 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
     if (!h->ws || P > h->cap_pairs || ntiles > h->cap_tiles) {
         if (h->ws) {
@@ -204,38 +199,34 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
     h->plan.ntiles = (uint32_t)ntiles;
     h->plan.c0 = kC0;
     h->plan.U = kU;
-    h->plan.policy = 0;
-    h->plan.spin_limit = g_ctx.spin_limit;
+    h->plan.stage_words = kStageWords;
     const uint64_t gp = std::min<uint64_t>((P + 1 + 255) / 256, 8192);
     h->grid_plan = (int)std::max<uint64_t>(gp, 1);
-    for (int co = 0; co < 2; co++) {
-        const int occ = std::max(1, g_ctx.occ_merge[(h->bsr ? 2 : 0) + co]);
-        const uint64_t g = std::min<uint64_t>((uint64_t)occ * g_ctx.num_cu, ntiles);
-        h->grid_main[co] = (int)std::max<uint64_t>(g, 1);
-    }
     return SETOPS_OK;
 }
 
 int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
 {
     size_t off = 0;
-    const size_t o_uoff = off;
-    off = align_up(off + (P + 1) * 8, 256);
-    const size_t o_tf = off;
-    off = align_up(off + ntiles * 4, 256);
-    const size_t o_tl = off;
-    off = align_up(off + ntiles * 4, 256);
-    const size_t o_st = off;
-    off = align_up(off + ntiles * 8, 256);
-    const size_t o_ga = off;
-    off = align_up(off + (ntiles + 1) * 8, 256);
-    const size_t o_gb = off;
-    off = align_up(off + (ntiles + 1) * 8, 256);
-    const size_t o_q = off;
-    off = align_up(off + 4 * kClaimStride * kClaimQueues + 256, 256);
-    // one pending area per resident workgroup of the widest main-kernel grid
-    const size_t o_pend = off;
-    off = align_up(off + (size_t)4 * pend_words<kU, kMaxSeg>() * (size_t)max_grid(), 256);
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t o_uoff = take((P + 1) * 8);
+    const size_t o_tf = take(ntiles * 4);
+    const size_t o_tl = take(ntiles * 4);
+    const size_t o_ga = take((ntiles + 1) * 8);
+    const size_t o_gb = take((ntiles + 1) * 8);
+    const size_t o_tc = take(ntiles * 4);
+    const size_t o_te = take(ntiles * 4);
+    const size_t o_to = take((ntiles + 1) * 8);
+    const size_t ngroups = (ntiles + kScanGroup - 1) / kScanGroup;
+    const size_t o_gc = take(ngroups * 8);
+    const size_t o_ge = take(ngroups * 8);
+    const size_t o_pl = take(P * 4 + 4);
+    const size_t o_misc = take(256);  // error word, single-pair offsets
+    const size_t o_stage = take(ntiles * (size_t)kStageWords * 4);
     void *ws = nullptr;
     if (hipMalloc(&ws, off) != hipSuccess) {
         (void)hipGetLastError();
@@ -243,23 +234,32 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
     }
     char *b = static_cast<char *>(ws);
     h->ws = ws;
+    h->ws_bytes = off;
     h->plan.uoff = reinterpret_cast<uint64_t *>(b + o_uoff);
     h->plan.tile_first = reinterpret_cast<uint32_t *>(b + o_tf);
     h->plan.tile_last = reinterpret_cast<uint32_t *>(b + o_tl);
-    h->plan.status = reinterpret_cast<uint64_t *>(b + o_st);
     h->plan.tile_ga = reinterpret_cast<uint64_t *>(b + o_ga);
     h->plan.tile_gb = reinterpret_cast<uint64_t *>(b + o_gb);
-    h->plan.queue = reinterpret_cast<uint32_t *>(b + o_q);
-    // claim counters occupy o_q .. o_q + 4*kClaimStride*kClaimQueues; error word and the
-    // single-pair offsets follow in the last 256 bytes
-    const size_t o_tail = o_q + 4 * kClaimStride * kClaimQueues;
-    h->d_error = reinterpret_cast<uint32_t *>(b + o_tail);
-    h->plan.single_off = reinterpret_cast<uint64_t *>(b + o_tail + 16);
-    h->plan.pend = reinterpret_cast<uint32_t *>(b + o_pend);
+    h->plan.tile_count = reinterpret_cast<uint32_t *>(b + o_tc);
+    h->plan.tile_elems = reinterpret_cast<uint32_t *>(b + o_te);
+    h->plan.tile_off = reinterpret_cast<uint64_t *>(b + o_to);
+    h->plan.group_cnt = reinterpret_cast<unsigned long long *>(b + o_gc);
+    h->plan.group_el = reinterpret_cast<unsigned long long *>(b + o_ge);
+    h->plan.pair_local = reinterpret_cast<uint32_t *>(b + o_pl);
+    h->d_error = reinterpret_cast<uint32_t *>(b + o_misc);
+    h->plan.single_off = reinterpret_cast<uint64_t *>(b + o_misc + 16);
+    h->plan.stage = reinterpret_cast<uint32_t *>(b + o_stage);
     return SETOPS_OK;
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <bool BSR, bool CO>
+void launch_merge(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO>), dim3(h->plan.ntiles), dim3(kNT), 0, stream, h->in,
+                       h->plan, out);
+}
 
 int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64_t cap, uint64_t *d_out_off,
                  uint32_t *d_counts, unsigned long long *d_totals, hipStream_t stream)
@@ -277,10 +277,12 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.counts = d_counts;
     out.totals = d_totals;
     out.error = h->d_error;
+    // plan: sizes only
     hipLaunchKernelGGL(plan_csr_kernel, dim3(h->grid_plan), dim3(256), 0, stream, h->in, h->plan, out);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(plan_split_kernel, dim3((h->plan.ntiles + 1 + 255) / 256), dim3(256), 0, stream, h->in, h->plan);
     HIP_TRY(hipGetLastError());
+    // count (and stage)
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (h->timing) {
         if (!h->spare.empty()) {
@@ -292,26 +294,33 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
         }
         HIP_TRY(hipEventRecord(ev.first, stream));
     }
-    const int g = h->grid_main[count_only ? 1 : 0];
     if (h->bsr) {
-        if (count_only)
-            hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, true, true>), dim3(g), dim3(kNT), 0, stream, h->in,
-                               h->plan, out);
-        else
-            hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, true, false>), dim3(g), dim3(kNT), 0, stream, h->in,
-                               h->plan, out);
+        if (count_only) launch_merge<true, true>(h, out, stream);
+        else launch_merge<true, false>(h, out, stream);
     } else {
-        if (count_only)
-            hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, false, true>), dim3(g), dim3(kNT), 0, stream, h->in,
-                               h->plan, out);
-        else
-            hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, false, false>), dim3(g), dim3(kNT), 0, stream, h->in,
-                               h->plan, out);
+        if (count_only) launch_merge<false, true>(h, out, stream);
+        else launch_merge<false, false>(h, out, stream);
     }
     HIP_TRY(hipGetLastError());
     if (h->timing) {
         HIP_TRY(hipEventRecord(ev.second, stream));
         h->events.push_back(ev);
+    }
+    // scan
+    const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
+    hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3((h->plan.ntiles + kScanGroup - 1) / kScanGroup),
+                       dim3(kScanGroup), 0, stream, h->plan, out);
+    HIP_TRY(hipGetLastError());
+    // write
+    if (!count_only || d_out_off) {
+        const uint32_t gp = (h->plan.ntiles + 3) / 4;  // 4 tiles (waves) per workgroup
+        if (h->bsr)
+            hipLaunchKernelGGL((place_tiles_kernel<256, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
+        else
+            hipLaunchKernelGGL((place_tiles_kernel<256, false>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
+        HIP_TRY(hipGetLastError());
     }
     h->last_stream = stream;
     return SETOPS_OK;
@@ -668,7 +677,7 @@ int setops_hip_batch_info(setops_hip_batch *handle, uint64_t *ntiles, uint64_t *
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
     if (ntiles) *ntiles = handle->plan.ntiles;
-    if (grid) *grid = (uint64_t)handle->grid_main[0];
+    if (grid) *grid = (uint64_t)handle->plan.ntiles;
     if (units) *units = handle->plan.utotal;
     return SETOPS_OK;
 }
