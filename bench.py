@@ -116,6 +116,70 @@ def plain_gpu(torch, A, a_off, B, b_off, sample, stream):
     return {"gelems": E_s / el / 1e9, "pairs": int(len(idx)), "us_per_pair": el / len(idx) * 1e6}
 
 
+def skewed_leg(torch, steps, cpu_threads, with_cpu):
+    """The skewed workload: one resident large set (2^24 over 2^28) against 4,096 sets of 256 with
+    26 planted matches each; the large set is read in place by every pair (gather batch).  Reported
+    on unique bytes: the large set once, the small sets once, outputs and counts once."""
+    from simd_set_operations_amd import datagen
+    from simd_set_operations_amd.batch import GatherBatch
+
+    L, S = datagen.skewed()
+    P, m = S.shape
+    pool = S.reshape(-1)
+    a_beg = np.arange(P, dtype=np.uint64) * m
+    a_end = a_beg + m
+    b_beg = np.zeros(P, np.uint64)
+    b_end = np.full(P, L.size, np.uint64)
+    shared = datagen.SKEWED["shared"]
+    E_unique = L.size + pool.size
+    out_total = P * shared
+    B_unique = 4 * E_unique + 4 * out_total + 4 * P
+    res = {"pairs": P, "large": int(L.size), "small": m, "elements_unique": E_unique,
+           "elements_streamed": int(P * (L.size + m)), "bytes_unique": B_unique}
+    dL = torch.from_numpy(L.view(np.int32)).cuda()
+    stream = torch.cuda.current_stream()
+    for algo in ("hip_galloping", "hip_binary_search"):
+        gb = GatherBatch(algo, pool, a_beg, a_end, dL, b_beg, b_end)
+        reps = steps
+        for _ in range(2):
+            gb.run(stream)
+        torch.cuda.synchronize()
+        _, cnt, _ = gb.results()
+        if not np.all(cnt == shared):
+            log(f"skewed {algo}: counts differ from the planted {shared}")
+            sys.exit(3)
+        gb.kernel_time()
+        gb.set_timing(True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            gb.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        k_ms, k_runs = gb.kernel_time()
+        k_ms /= max(k_runs, 1)
+        gb.close()
+        res[algo] = {"ms_per_batch": ms, "kernel_ms": k_ms, "gelems_unique": E_unique / (ms / 1e3) / 1e9,
+                     "hbm_frac_unique": B_unique / (ms / 1e3) / (HBM_PEAK_GBS * 1e9)}
+        log(f"skewed {algo}: {res[algo]}")
+    if with_cpu:
+        import oracle
+
+        t0 = time.perf_counter()
+        cnt, _, _ = oracle.batch_intersect_gather("galloping", pool, a_beg, a_end, L, b_beg, b_end,
+                                                  nthreads=cpu_threads, with_output=True)
+        el = time.perf_counter() - t0
+        if not np.all(cnt == shared):
+            log("skewed cpu galloping: counts differ")
+            sys.exit(3)
+        res["cpu_port_galloping"] = {"ms_per_batch": el * 1e3, "threads": cpu_threads,
+                                     "gelems_unique": E_unique / el / 1e9}
+        log(f"skewed cpu: {res['cpu_port_galloping']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +189,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=6.0, help="seconds of CPU baseline timing")
     ap.add_argument("--plain-sample", type=int, default=2048)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-skewed", action="store_true", help="skip the skewed resident-set leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times) and exit: the reference kernel of "
+                         "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,6 +201,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
 
     from simd_set_operations_amd import datagen, intersect
     from simd_set_operations_amd.batch import DeviceBatch
@@ -220,6 +298,8 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_skewed:
+        extra["skewed"] = skewed_leg(torch, args.steps, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines)
 
     rec = {
         "metric": "batched intersect Gelem/s + %HBM roofline; p50/p99 query latency vs plain",

@@ -119,6 +119,16 @@ int setops_hip_batch_create(setops_hip_batch **handle, int algo, const uint32_t 
                             uint64_t a_len, const uint64_t *d_a_off, const uint32_t *d_b, const uint32_t *d_b_states,
                             uint64_t b_len, const uint64_t *d_b_off, uint64_t npairs);
 
+/* Gather form: pair p is (A[a_beg[p] .. a_end[p]), B[b_beg[p] .. b_end[p])) anywhere in the pools
+ * (pairs may overlap and share sets: graph edges over one CSR, queries over one corpus).  Each
+ * set must hold < 2^32 elements.  Creation counts the batch's work on the device and waits for it
+ * (one synchronisation); runs do not synchronise.  Same outputs and algorithms as the CSR form;
+ * the reference's harness builds such pairs from datafile sets, benchmark/src/timer/harness.rs:81-172. */
+int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const uint32_t *d_a,
+                                   const uint32_t *d_a_states, uint64_t a_len, const uint64_t *d_a_beg,
+                                   const uint64_t *d_a_end, const uint32_t *d_b, const uint32_t *d_b_states,
+                                   uint64_t b_len, const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs);
+
 /* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
  *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
  *                          both NULL selects count-only mode (out_cap ignored)

@@ -92,6 +92,9 @@ def lib():
         L.oracle_svs.argtypes = [_c.c_int, _vp, _vp, _sz, _vp, _vp, _c.POINTER(_vp)]
         L.oracle_batch_intersect.restype = _c.c_int
         L.oracle_batch_intersect.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]
+        L.oracle_batch_intersect_gather.restype = _c.c_int
+        L.oracle_batch_intersect_gather.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
+                                                    _vp]
         _lib = L
     return _lib
 
@@ -205,6 +208,25 @@ def batch_intersect(algo, A, a_off, B, b_off, nthreads=1, with_output=True):
     counts = np.zeros(P, np.uint32)
     rc = lib().oracle_batch_intersect(code, nthreads, _p(A), _p(a_off), _p(B), _p(b_off), P, _p(cap_off),
                                       _p(out) if with_output else None, _p(counts))
+    if rc != 0:
+        raise RuntimeError("oracle batch failed")
+    return counts, cap_off, out
+
+
+def batch_intersect_gather(algo, A, a_beg, a_end, B, b_beg, b_end, nthreads=1, with_output=True):
+    """One intersection per host thread over gather pairs (A[a_beg[p]:a_end[p]], B[b_beg[p]:b_end[p]]).
+    Returns (counts, cap_off, strided out)."""
+    code = ALGOS[algo] if isinstance(algo, str) else algo
+    A, B = _u32(A), _u32(B)
+    a_beg, a_end, b_beg, b_end = (np.ascontiguousarray(x, np.uint64) for x in (a_beg, a_end, b_beg, b_end))
+    P = a_beg.size
+    caps = np.minimum(a_end - a_beg, b_end - b_beg)
+    cap_off = np.zeros(P + 1, np.uint64)
+    np.cumsum(caps, out=cap_off[1:])
+    out = np.zeros(max(int(cap_off[-1]), 1), np.uint32) if with_output else None
+    counts = np.zeros(P, np.uint32)
+    rc = lib().oracle_batch_intersect_gather(code, nthreads, _p(A), _p(a_beg), _p(a_end), _p(B), _p(b_beg), _p(b_end),
+                                             P, _p(cap_off), _p(out) if with_output else None, _p(counts))
     if rc != 0:
         raise RuntimeError("oracle batch failed")
     return counts, cap_off, out

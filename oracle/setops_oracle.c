@@ -448,15 +448,15 @@ size_t oracle_svs(int algo, const uint32_t *const *sets, const size_t *lens, siz
 }
 
 /* ------------------------------------------------------------------------------------------------
- * Batch runner: one intersection per host thread over CSR-laid-out pairs (the CPU baseline).
- * Pair p is (A[a_off[p]..a_off[p+1]), B[b_off[p]..b_off[p+1])).  Outputs go to a capacity-strided
+ * Batch runner: one intersection per host thread (the CPU baseline).  Pair p is
+ * (A[a_beg[p]..a_end[p]), B[b_beg[p]..b_end[p])) -- CSR batches pass end = beg + 1.  Outputs go to a capacity-strided
  * buffer (pair p at cap_off[p], capacity min(|A|,|B|)), counts[p] receives the length.
  * Pairs are handed out dynamically in chunks of 64 so uneven sizes balance across threads.
  * -------------------------------------------------------------------------------------------- */
 typedef struct {
     int algo;
     const uint32_t *A, *B;
-    const uint64_t *a_off, *b_off, *cap_off;
+    const uint64_t *a_beg, *a_end, *b_beg, *b_end, *cap_off;
     uint32_t *out;
     uint32_t *counts;
     size_t npairs;
@@ -475,8 +475,8 @@ static void *batch_worker(void *arg)
         if (p0 >= job->npairs) break;
         size_t p1 = p0 + 64 < job->npairs ? p0 + 64 : job->npairs;
         for (size_t p = p0; p < p1; p++) {
-            const uint32_t *a = job->A + job->a_off[p], *b = job->B + job->b_off[p];
-            size_t na = job->a_off[p + 1] - job->a_off[p], nb = job->b_off[p + 1] - job->b_off[p];
+            const uint32_t *a = job->A + job->a_beg[p], *b = job->B + job->b_beg[p];
+            size_t na = job->a_end[p] - job->a_beg[p], nb = job->b_end[p] - job->b_beg[p];
             uint32_t *o = job->out ? job->out + job->cap_off[p] : NULL;
             job->counts[p] = (uint32_t)oracle_intersect2(job->algo, a, na, b, nb, o);
         }
@@ -484,16 +484,22 @@ static void *batch_worker(void *arg)
     return NULL;
 }
 
-int oracle_batch_intersect(int algo, int nthreads, const uint32_t *A, const uint64_t *a_off,
-                           const uint32_t *B, const uint64_t *b_off, size_t npairs,
-                           const uint64_t *cap_off, uint32_t *out, uint32_t *counts)
+/* Gather batch: pair p is (A[a_beg[p] .. a_end[p]), B[b_beg[p] .. b_end[p])).  Results of pair p go
+ * to out + cap_off[p] (out may be NULL: counts only); pairs are shared out to `nthreads` threads in
+ * chunks of 64. */
+int oracle_batch_intersect_gather(int algo, int nthreads, const uint32_t *A, const uint64_t *a_beg,
+                                  const uint64_t *a_end, const uint32_t *B, const uint64_t *b_beg,
+                                  const uint64_t *b_end, size_t npairs, const uint64_t *cap_off, uint32_t *out,
+                                  uint32_t *counts)
 {
     batch_job job;
     job.algo = algo;
     job.A = A;
     job.B = B;
-    job.a_off = a_off;
-    job.b_off = b_off;
+    job.a_beg = a_beg;
+    job.a_end = a_end;
+    job.b_beg = b_beg;
+    job.b_end = b_end;
     job.cap_off = cap_off;
     job.out = out;
     job.counts = counts;
@@ -508,4 +514,13 @@ int oracle_batch_intersect(int algo, int nthreads, const uint32_t *A, const uint
     free(th);
     pthread_mutex_destroy(&job.lock);
     return 0;
+}
+
+/* CSR batch: pair p is (A[a_off[p] .. a_off[p+1]), B[b_off[p] .. b_off[p+1])). */
+int oracle_batch_intersect(int algo, int nthreads, const uint32_t *A, const uint64_t *a_off,
+                           const uint32_t *B, const uint64_t *b_off, size_t npairs,
+                           const uint64_t *cap_off, uint32_t *out, uint32_t *counts)
+{
+    return oracle_batch_intersect_gather(algo, nthreads, A, a_off, a_off + 1, B, b_off, b_off + 1, npairs, cap_off,
+                                         out, counts);
 }

@@ -71,6 +71,10 @@ HIP_SIGNATURES = {
         _c.c_int,
         [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _c.c_uint64, _vp, _c.c_uint64],
     ),
+    "setops_hip_batch_create_gather": (
+        _c.c_int,
+        [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp, _c.c_uint64],
+    ),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),
     "setops_hip_batch_set_timing": (_c.c_int, [_vp, _c.c_int]),
@@ -88,6 +92,10 @@ DATAGEN_SIGNATURES = {
     "setops_gen_uneven_sizes": (
         _c.c_int,
         [_c.c_uint64, _c.c_uint64, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _vp, _vp],
+    ),
+    "setops_gen_skewed": (
+        _c.c_int,
+        [_c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _vp, _vp],
     ),
     "setops_gen_uneven_fill": (
         _c.c_int,

@@ -2,7 +2,9 @@
 
 Device memory comes from PyTorch (plumbing only); the library receives raw pointers through the
 C ABI (``setops_hip_batch_*``).  A :class:`DeviceBatch` uploads a CSR batch once, plans once
-(sizes only), and can be run many times on a stream.
+(sizes only), and can be run many times on a stream.  A :class:`GatherBatch` takes pairs as
+(begin, end) index ranges into two pools, so pairs can share sets (graph edges over one CSR,
+queries over one corpus).
 """
 import ctypes
 
@@ -21,20 +23,40 @@ def _torch():
     return torch
 
 
+def _is_tensor(x):
+    return type(x).__module__.startswith("torch")
+
+
 def to_device_u32(arr, torch):
     """Upload a uint32 numpy array (as int32 bits); returns a tensor whose storage is 16-byte
-    aligned (torch allocations are 256-byte aligned)."""
+    aligned (torch allocations are 256-byte aligned).  A device tensor is used as it is."""
+    if _is_tensor(arr):
+        return arr
     a = np.ascontiguousarray(arr, dtype=np.uint32)
     t = torch.from_numpy(a.view(np.int32)) if a.size else torch.zeros(4, dtype=torch.int32)
     return t.to("cuda")
 
 
 def to_device_u64(arr, torch):
+    if _is_tensor(arr):
+        return arr
     a = np.ascontiguousarray(arr, dtype=np.uint64)
     return torch.from_numpy(a.view(np.int64)).to("cuda")
 
 
-class DeviceBatch:
+class _Batch:
+    def _outputs(self, capacity):
+        torch = self.torch
+        self.capacity = int(capacity)
+        cap = max(self.capacity, 4)
+        self.d_out = torch.empty(cap, dtype=torch.int32, device="cuda")
+        self.d_out_s = torch.empty(cap, dtype=torch.int32, device="cuda") if self.bsr else None
+        self.d_out_off = torch.empty(self.P + 1, dtype=torch.int64, device="cuda")
+        self.d_counts = torch.empty(self.P, dtype=torch.int32, device="cuda")
+        self.d_totals = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+
+class DeviceBatch(_Batch):
     """CSR batch on the GPU: pair p = (A[a_off[p]:a_off[p+1]], B[b_off[p]:b_off[p+1]])."""
 
     def __init__(self, algo, A, a_off, B, b_off, A_states=None, B_states=None):
@@ -75,13 +97,7 @@ class DeviceBatch:
             )
         )
         self.handle = h
-        self.capacity = int(np.minimum(self.a_sizes, self.b_sizes).sum())
-        cap = max(self.capacity, 4)
-        self.d_out = torch.empty(cap, dtype=torch.int32, device="cuda")
-        self.d_out_s = torch.empty(cap, dtype=torch.int32, device="cuda") if self.bsr else None
-        self.d_out_off = torch.empty(self.P + 1, dtype=torch.int64, device="cuda")
-        self.d_counts = torch.empty(self.P, dtype=torch.int32, device="cuda")
-        self.d_totals = torch.zeros(2, dtype=torch.int64, device="cuda")
+        self._outputs(np.minimum(self.a_sizes, self.b_sizes).sum())
 
     # -- running ----------------------------------------------------------------------------
     def run(self, stream=None, count_only=False):
@@ -148,3 +164,54 @@ class DeviceBatch:
             self.close()
         except Exception:
             pass
+
+
+class GatherBatch(DeviceBatch):
+    """Gather batch on the GPU: pair p = (A[a_beg[p]:a_end[p]], B[b_beg[p]:b_end[p]]).
+
+    Pools and index arrays may be numpy arrays (uploaded) or device tensors (used in place).
+    ``capacity`` sizes the output (default: sum of min(|A_p|, |B_p|), computed on the host when
+    the index arrays are numpy).  ``outputs=False`` allocates no result buffer (count-only use)."""
+
+    def __init__(self, algo, A, a_beg, a_end, B, b_beg, b_end, A_states=None, B_states=None, capacity=None,
+                 outputs=True):
+        torch = _torch()
+        from .intersect import algo_id
+
+        lib = _native.hip()
+        self.torch = torch
+        self.algo = algo
+        self.P = int(len(a_beg))
+        if not (len(a_end) == len(b_beg) == len(b_end) == self.P) or self.P == 0:
+            raise ValueError("a_beg, a_end, b_beg, b_end must have the same non-zero length")
+        if capacity is None:
+            if _is_tensor(a_beg):
+                capacity = int(torch.minimum(a_end - a_beg, b_end - b_beg).sum().item())
+            else:
+                la = np.asarray(a_end, np.int64) - np.asarray(a_beg, np.int64)
+                lb = np.asarray(b_end, np.int64) - np.asarray(b_beg, np.int64)
+                if np.any(la < 0) or np.any(lb < 0):
+                    raise ValueError("end must not precede begin")
+                capacity = int(np.minimum(la, lb).sum())
+        self.dA = to_device_u32(A, torch)
+        self.dB = to_device_u32(B, torch)
+        self.a_len = int(self.dA.numel()) if len(A) else 0
+        self.b_len = int(self.dB.numel()) if len(B) else 0
+        self.dA_beg = to_device_u64(a_beg, torch)
+        self.dA_end = to_device_u64(a_end, torch)
+        self.dB_beg = to_device_u64(b_beg, torch)
+        self.dB_end = to_device_u64(b_end, torch)
+        self.bsr = A_states is not None
+        self.dAs = to_device_u32(A_states, torch) if self.bsr else None
+        self.dBs = to_device_u32(B_states, torch) if self.bsr else None
+        h = _c.c_void_p()
+        _native.check(
+            lib.setops_hip_batch_create_gather(
+                _c.byref(h), algo_id(algo), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
+                self.a_len, self.dA_beg.data_ptr(), self.dA_end.data_ptr(), self.dB.data_ptr(),
+                self.dBs.data_ptr() if self.bsr else None, self.b_len, self.dB_beg.data_ptr(),
+                self.dB_end.data_ptr(), self.P,
+            )
+        )
+        self.handle = h
+        self._outputs(capacity if outputs else 0)

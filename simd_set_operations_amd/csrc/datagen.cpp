@@ -161,4 +161,44 @@ int setops_gen_uneven_fill(uint64_t seed, uint64_t npairs, double lmin, double l
     return bad ? -1 : 0;
 }
 
+// Skewed batch: one large set of `large_n` distinct values from [0, universe), sorted, and
+// `nsmall` small sets of `small_n` values each: `shared` drawn (distinct) from the large set, the
+// rest from its complement; each small set sorted.  small[i*small_n ...] is set i.
+int setops_gen_skewed(uint64_t seed, uint64_t universe, uint64_t large_n, uint64_t nsmall, uint64_t small_n,
+                      uint64_t shared, uint32_t *large, uint32_t *small)
+{
+    if (large_n + small_n > universe || shared > small_n || shared > large_n) return -1;
+    std::vector<uint8_t> in_large(universe / 8 + 1, 0);
+    {
+        SplitMix64 rng(stream_seed(seed, 0));
+        draw_distinct(rng, universe, large_n, in_large, large);
+        for (uint64_t k = 0; k < large_n; k++) in_large[large[k] >> 3] |= (uint8_t)(1u << (large[k] & 7));
+        std::sort(large, large + large_n);
+    }
+    int bad = 0;
+#pragma omp parallel reduction(| : bad)
+    {
+        std::vector<uint8_t> seen_idx(large_n / 8 + 1, 0);
+        std::vector<uint32_t> idx(shared);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < (int64_t)nsmall; i++) {
+            SplitMix64 rng(stream_seed(seed, 1 + (uint64_t)i));
+            uint32_t *out = small + (uint64_t)i * small_n;
+            draw_distinct(rng, large_n, shared, seen_idx, idx.data());
+            for (uint64_t k = 0; k < shared; k++) out[k] = large[idx[k]];
+            // the rest: outside the large set and distinct (a small set, so a linear check)
+            for (uint64_t k = shared; k < small_n;) {
+                const uint32_t v = (uint32_t)rng.below(universe);
+                if (in_large[v >> 3] & (1u << (v & 7))) continue;
+                bool dup = false;
+                for (uint64_t j = shared; j < k; j++) dup |= out[j] == v;
+                if (dup) continue;
+                out[k++] = v;
+            }
+            std::sort(out, out + small_n);
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 } // extern "C"

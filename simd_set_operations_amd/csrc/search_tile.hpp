@@ -1,0 +1,312 @@
+// search_tile.hpp -- probe-per-element tile kernel for skewed pairs (gfx950).
+//
+// The search policy of the reference's galloping / binary_search_intersect
+// (setops/src/intersect/galloping.rs:7-49): every element of the smaller set ("probe") is looked
+// up in the larger set.  Units are probes (tile_plan.hpp, POLICY_SEARCH), so a tile carries
+// U = NT*KS probes whatever the sizes of the larger sets.  Per tile:
+//   1. segment table; stage the probe windows in LDS (the smaller side of each pair, which may
+//      be A or B -- orientation does not change the result, A∩B ascending),
+//   2. narrow each pair's larger set to [lo, hi): lower bound of the segment's first probe and
+//      upper bound of its last probe, by a wave-cooperative 64-way partition search,
+//   3. stage the narrowed larger windows in LDS while they fit (a prefix of the segments); the
+//      rest are searched in global memory,
+//   4. each lane looks up its KS probes, in order, from where the previous probe landed:
+//      galloping (exponential then binary, galloping.rs:7-30) or binary search over the rest of
+//      the window (binary_search_intersect, galloping.rs:32-49),
+//   5. block scan of hits, results (probe values; BSR: base and state AND) staged in order,
+//      tile and per-pair counts -- the same outputs as the merge tile kernel.
+#pragma once
+
+#include "tile_plan.hpp"
+
+namespace setops {
+
+template <int NT, int KS, int MAXSEG, int CAPB, bool BSR>
+struct SearchSmem {
+    static constexpr int U = NT * KS;
+    static constexpr int PKEYS = U + 8 * MAXSEG + 16;
+    static constexpr int LKEYS = CAPB + 16;
+    uint32_t pkey[PKEYS];               // probe windows
+    uint32_t pst[BSR ? PKEYS : 4];
+    uint32_t lkey[LKEYS];               // staged larger windows
+    uint32_t lst[BSR ? LKEYS : 4];
+    uint32_t seg_u[MAXSEG + 1];
+    uint32_t seg_la[MAXSEG];            // LDS index of the segment's first probe
+    uint32_t seg_wa[MAXSEG];            // probes in the window
+    uint32_t seg_lo[MAXSEG];            // narrowed larger window [lo, hi), pair-relative
+    uint32_t seg_hi[MAXSEG];
+    uint32_t seg_lb[MAXSEG];            // LDS index of lkey[lo], or kNotStaged
+    uint32_t seg_flags[MAXSEG];         // bit0: pair starts here, bit1: ends here, bit2: larger side is A
+    uint64_t seg_lbase[MAXSEG];         // pool index of the larger set's first element
+    uint32_t lane_mask[NT];
+    uint32_t lane_pfx[NT];
+    uint32_t scan_scratch[NT / kWave];
+    uint32_t wave_el[NT / kWave];
+    uint32_t win_c[MAXSEG + 1];
+    uint64_t win_g[MAXSEG];
+    uint32_t win_src[MAXSEG];
+};
+
+constexpr uint32_t kNotStaged = 0xFFFFFFFFu;
+
+// Lower bound of x in arr[lo, hi) starting from lo: galloping (probe lo+1, lo+3, lo+7, ... then
+// binary search in the last gap) or plain binary search.  Works on LDS or global arrays.
+template <bool GALLOP, class Arr>
+__device__ __forceinline__ uint32_t lower_bound_from(const Arr &arr, uint32_t lo, uint32_t hi, uint32_t x)
+{
+    if constexpr (GALLOP) {
+        uint32_t step = 1, prev = lo;
+        uint32_t cur = lo;
+        while (cur < hi && arr(cur) < x) {
+            prev = cur + 1;
+            cur = (step >= hi - lo) ? hi : lo + step;
+            step <<= 1;
+        }
+        lo = prev;
+        hi = cur;  // arr[prev-1] < x <= arr[cur] (or cur == hi)
+    }
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (arr(mid) < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, bool GALLOP>
+__global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
+{
+    static_assert(KS >= 1 && KS <= 31, "lane masks hold KS bits");
+    constexpr int U = NT * KS;
+    constexpr int NW = NT / kWave;
+    using Smem = SearchSmem<NT, KS, MAXSEG, CAPB, BSR>;
+    __shared__ Smem sm;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = tid / kWave;
+
+    {
+        // one tile per workgroup (a loop over tiles here costs ~20% on the merge kernel)
+        const uint32_t t = tile0 + blockIdx.x;
+        if (t >= plan.ntiles) return;
+        // ---- 1. segment table, probe windows -------------------------------------------------------
+        const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
+        const uint64_t u0 = (uint64_t)t * U;
+        uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
+        uint32_t nseg = p_last - p_first + 1;
+        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG;
+        if (bad) nseg = 0;
+        uint32_t cp = 0;
+        uint64_t gp = 0;
+        uint32_t swap = 0;
+        if (tid < nseg) {
+            const SegWin w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            swap = (w.nb < w.na) ? 1u : 0u;  // probes come from the smaller side
+            gp = (swap ? w.b : w.a) + w.i0;
+            const uint32_t n = (uint32_t)(w.i1 - w.i0);
+            cp = n ? ((uint32_t)(gp & 3) + n + 3) / 4 : 0u;
+            sm.seg_u[tid] = w.u;
+            sm.seg_wa[tid] = n;
+            sm.seg_flags[tid] = w.flags | (swap << 2);
+            sm.seg_lbase[tid] = swap ? w.a : w.b;
+            sm.seg_hi[tid] = (uint32_t)(swap ? w.na : w.nb);  // larger size until narrowed
+        }
+        uint32_t tot = 0;
+        const uint32_t pfx = block_exclusive_scan<NT>(cp, sm.scan_scratch, &tot);
+        if (4u * tot + 8u > (uint32_t)Smem::PKEYS) bad = true;  // not reachable by the unit bound
+        if (tid < nseg) {
+            sm.win_c[tid] = pfx;
+            sm.win_g[tid] = gp & ~3ull;
+            sm.win_src[tid] = swap;
+            sm.seg_la[tid] = 4u * pfx + (uint32_t)(gp & 3);
+        }
+        if (tid == 0) {
+            sm.win_c[nseg] = tot;
+            sm.seg_u[nseg] = ulen;
+        }
+        __syncthreads();
+        if (bad) {
+            nseg = 0;
+            ulen = 0;
+            if (tid == 0 && out.error) atomicOr(out.error, 2u);
+        } else {
+            stage_windows<NT>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey);
+            if constexpr (BSR)
+                stage_windows<NT>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst);
+        }
+        __syncthreads();
+
+        // ---- 2. narrow the larger sets: one wave per segment ----------------------------------------
+        for (uint32_t s = wave; s < nseg; s += NW) {
+            const uint32_t n = sm.seg_wa[s];
+            const uint32_t nl = sm.seg_hi[s];
+            uint32_t lo = 0, hi = 0;
+            if (n) {
+                const bool large_is_a = (sm.seg_flags[s] >> 2) & 1u;
+                const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
+                const uint32_t vf = sm.pkey[sm.seg_la[s]], vl = sm.pkey[sm.seg_la[s] + n - 1];
+                lo = (uint32_t)group_partition_point<kWave>(0, nl, lane, 0, [&](uint64_t x) { return L[x] < vf; });
+                hi = (uint32_t)group_partition_point<kWave>(lo, nl, lane, 0, [&](uint64_t x) { return L[x] <= vl; });
+            }
+            if (lane == 0) {
+                sm.seg_lo[s] = lo;
+                sm.seg_hi[s] = hi;
+            }
+        }
+        __syncthreads();
+
+        // ---- 3. stage the narrowed larger windows while they fit --------------------------------------
+        {
+            uint32_t cl = 0;
+            uint64_t gl = 0;
+            uint32_t src = 0;
+            if (tid < nseg) {
+                const uint32_t n = sm.seg_hi[tid] - sm.seg_lo[tid];
+                gl = sm.seg_lbase[tid] + sm.seg_lo[tid];
+                src = ((sm.seg_flags[tid] >> 2) & 1u) ? 0u : 1u;  // pool of the larger side
+                cl = n ? ((uint32_t)(gl & 3) + n + 3) / 4 : 0u;
+            }
+            uint32_t ltot = 0;
+            const uint32_t lpfx = block_exclusive_scan<NT>(cl, sm.scan_scratch, &ltot);
+            const bool fits = lpfx + cl <= (uint32_t)(CAPB / 4);
+            // staged segments form a prefix; their count is the number of fitting threads
+            uint32_t nst = 0;
+            {
+                const uint64_t bal = ballot(tid < nseg && fits);
+                if (lane == 0) sm.wave_el[wave] = (uint32_t)__popcll(bal);
+            }
+            __syncthreads();
+            for (int k = 0; k < NW; k++) nst += sm.wave_el[k];
+            if (tid < nseg) {
+                if (fits) {
+                    sm.win_c[tid] = lpfx;
+                    sm.win_g[tid] = gl & ~3ull;
+                    sm.win_src[tid] = src;
+                    sm.seg_lb[tid] = 4u * lpfx + (uint32_t)(gl & 3);
+                } else {
+                    sm.seg_lb[tid] = kNotStaged;
+                }
+                if (tid + 1 == nst) sm.win_c[nst] = lpfx + cl;
+            }
+            if (tid == 0 && nst == 0) sm.win_c[0] = 0;
+            __syncthreads();
+            if (nst) {
+                stage_windows<NT>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst, sm.lkey);
+                if constexpr (BSR)
+                    stage_windows<NT>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst,
+                                      sm.lst);
+            }
+            __syncthreads();
+        }
+
+        // ---- 4. lookups --------------------------------------------------------------------------
+        const uint32_t ub = tid * KS;
+        uint32_t hmask = 0, nelems = 0;
+        uint32_t svals[BSR ? KS : 1];
+        uint32_t s0 = 0;
+        if (ub < ulen) {
+            uint32_t lo = 0, hi = nseg;  // largest s with seg_u[s] <= ub
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sm.seg_u[mid] <= ub) lo = mid; else hi = mid;
+            }
+            s0 = lo;
+            uint32_t s = s0;
+            uint32_t cur = sm.seg_lo[s];  // pair-relative position in the larger set
+            for (int k = 0; k < KS; k++) {
+                const uint32_t u = ub + (uint32_t)k;
+                if (u >= ulen) break;
+                if (u >= sm.seg_u[s + 1]) {
+                    s++;
+                    cur = sm.seg_lo[s];
+                }
+                const uint32_t e = u - sm.seg_u[s];
+                if (e >= sm.seg_wa[s]) continue;  // padding unit
+                const uint32_t x = sm.pkey[sm.seg_la[s] + e];
+                const uint32_t lo_s = sm.seg_lo[s], hi_s = sm.seg_hi[s], lb = sm.seg_lb[s];
+                const bool large_is_a = (sm.seg_flags[s] >> 2) & 1u;
+                uint32_t pos;
+                bool found;
+                uint32_t lstate = 0;
+                if (lb != kNotStaged) {
+                    auto arr = [&](uint32_t i) { return sm.lkey[lb + (i - lo_s)]; };
+                    pos = lower_bound_from<GALLOP>(arr, cur, hi_s, x);
+                    found = pos < hi_s && arr(pos) == x;
+                    if constexpr (BSR)
+                        if (found) lstate = sm.lst[lb + (pos - lo_s)];
+                } else {
+                    const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
+                    auto arr = [&](uint32_t i) { return L[i]; };
+                    pos = lower_bound_from<GALLOP>(arr, cur, hi_s, x);
+                    found = pos < hi_s && L[pos] == x;
+                    if constexpr (BSR)
+                        if (found) lstate = (large_is_a ? in.As : in.Bs)[sm.seg_lbase[s] + pos];
+                }
+                cur = pos;
+                if constexpr (BSR) {
+                    const uint32_t stv = found ? (sm.pst[sm.seg_la[s] + e] & lstate) : 0u;
+                    found = stv != 0;
+                    svals[k] = stv;
+                    nelems += __popc(stv);
+                }
+                hmask |= found ? (1u << k) : 0u;
+            }
+        }
+        const uint32_t nwords = __popc(hmask);
+        if constexpr (!BSR) nelems = nwords;
+
+        // ---- 5. tile scan, staged outputs, counts ------------------------------------------------------
+        const uint32_t pfx2 = block_exclusive_scan<NT>(nwords, sm.scan_scratch, &tot);
+        sm.lane_mask[tid] = hmask;
+        sm.lane_pfx[tid] = pfx2;
+        if constexpr (BSR) {
+            uint32_t el = nelems;
+#pragma unroll
+            for (int dd = 1; dd < kWave; dd <<= 1) el += __shfl_xor(el, dd, kWave);
+            if (lane == 0) sm.wave_el[wave] = el;
+        }
+        __syncthreads();
+        if constexpr (!COUNT_ONLY) {
+            uint32_t *dk = plan.stage + (uint64_t)t * plan.stage_words;
+            uint32_t *ds = dk + plan.stage_words / 2;
+            uint32_t pos = pfx2;
+            uint32_t s = s0;
+#pragma unroll
+            for (int k = 0; k < KS; k++) {
+                const uint32_t u = ub + (uint32_t)k;
+                if (hmask & (1u << k)) {
+                    while (u >= sm.seg_u[s + 1]) s++;
+                    dk[pos] = sm.pkey[sm.seg_la[s] + (u - sm.seg_u[s])];
+                    if constexpr (BSR) ds[pos] = svals[k];
+                    pos++;
+                }
+            }
+        }
+        if (tid == 0) {
+            uint32_t el = tot;
+            if constexpr (BSR) {
+                el = 0;
+                for (int k = 0; k < NW; k++) el += sm.wave_el[k];
+            }
+            plan.tile_count[t] = tot;
+            plan.tile_elems[t] = el;
+        }
+        if (tid < nseg) {
+            auto before = [&](uint32_t x) -> uint32_t {
+                if (x >= ulen) return tot;
+                const uint32_t l = x / KS, k = x % KS;
+                return sm.lane_pfx[l] + __popc(sm.lane_mask[l] & ((1u << k) - 1u));
+            };
+            const uint32_t c0v = before(sm.seg_u[tid]);
+            const uint32_t c = before(sm.seg_u[tid + 1]) - c0v;
+            const uint32_t f = sm.seg_flags[tid];
+            const uint64_t p = (uint64_t)p_first + tid;
+            if (f & 1u) plan.pair_local[p] = c0v;
+            if (out.counts) {
+                if ((f & 3u) == 3u) out.counts[p] = c;
+                else if (c) atomicAdd(&out.counts[p], c);
+            }
+        }
+    }
+}
+
+} // namespace setops

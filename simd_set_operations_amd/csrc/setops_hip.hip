@@ -16,6 +16,7 @@
 
 #include "../../include/setops_hip.h"
 #include "merge_tile.hpp"
+#include "search_tile.hpp"
 
 using namespace setops;
 
@@ -48,14 +49,23 @@ int fail(int code, const char *fmt, ...)
 // ---------------------------------------------------------------------------------------------
 // kernel configuration
 // ---------------------------------------------------------------------------------------------
+// merge policy: 256 lanes x 16 merge-path steps per tile
 constexpr int kNT = 256;
 constexpr int kK = 16;
 constexpr int kU = kNT * kK;
 constexpr int kMaxSeg = 130;
 // every pair owns at least c0 units so a tile touches at most kMaxSeg pairs
 constexpr uint32_t kC0 = (kU + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
+// search policy: 256 lanes x 2 probes per tile, larger windows staged up to kCapB words
+constexpr int kKS = 2;
+constexpr int kUS = kNT * kKS;
+constexpr int kCapB = 3072;
+constexpr uint32_t kC0S = (kUS + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
+static_assert(kC0 >= (uint32_t)kK && kC0S >= (uint32_t)kKS, "a lane crosses at most one pair boundary");
 
-enum Kind { KIND_MERGE = 0 };
+constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
+
+enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2 };
 
 struct AlgoEntry {
     const char *name;
@@ -69,6 +79,9 @@ struct AlgoEntry {
 const AlgoEntry kAlgos[] = {
     {"hip_branchless_merge", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_MERGE, false},
     {"hip_branchless_merge_bsr", SETOPS_ALGO_BSR, KIND_MERGE, true},
+    {"hip_galloping", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_GALLOP, false},
+    {"hip_galloping_bsr", SETOPS_ALGO_BSR, KIND_GALLOP, true},
+    {"hip_binary_search", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_BINARY, false},
 };
 constexpr int kNumAlgos = (int)(sizeof(kAlgos) / sizeof(kAlgos[0]));
 
@@ -80,22 +93,15 @@ struct Context {
     int device = -1;
     int num_cu = 0;
     hipStream_t stream = nullptr;
-    int occ_merge[4] = {0, 0, 0, 0};  // [bsr][count_only] resident blocks per CU
-    std::mutex host_mu;               // serialises the host-pointer entry points
+    std::mutex host_mu;  // serialises the host-pointer entry points
     // grow-only device buffers for the host-pointer entry points
     void *host_buf = nullptr;
     size_t host_buf_bytes = 0;
-    setops_hip_batch *single[2] = {nullptr, nullptr};  // cached 1-pair handles [bsr]
+    setops_hip_batch *single[kNumAlgos] = {};  // cached 1-pair handles per algorithm
 };
 
 Context g_ctx;
 std::mutex g_init_mu;
-
-template <bool BSR, bool CO>
-void *merge_kernel_ptr()
-{
-    return reinterpret_cast<void *>(&merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO>);
-}
 
 int ensure_init(int device)
 {
@@ -117,15 +123,6 @@ int ensure_init(int device)
                     prop.gcnArchName);
     g_ctx.num_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
-    int occ = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)merge_kernel_ptr<false, false>(), kNT, 0));
-    g_ctx.occ_merge[0] = occ;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)merge_kernel_ptr<false, true>(), kNT, 0));
-    g_ctx.occ_merge[1] = occ;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)merge_kernel_ptr<true, false>(), kNT, 0));
-    g_ctx.occ_merge[2] = occ;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)merge_kernel_ptr<true, true>(), kNT, 0));
-    g_ctx.occ_merge[3] = occ;
     g_ctx.device = device;
     g_ctx.ready = true;
     return SETOPS_OK;
@@ -152,16 +149,20 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // ---------------------------------------------------------------------------------------------
 struct setops_hip_batch {
     int algo = 0;
+    Kind kind = KIND_MERGE;
     bool bsr = false;
+    bool gather = false;
     PairBatch in{};
     TilePlan plan{};
     uint64_t P = 0;
-    int grid_plan = 1;
     void *ws = nullptr;
     size_t ws_bytes = 0;
     uint64_t cap_pairs = 0;  // workspace capacity (pairs, tiles)
     uint64_t cap_tiles = 0;
     uint32_t *d_error = nullptr;
+    uint32_t *stage = nullptr;  // result staging (separate allocation, see stage_reserve)
+    uint64_t stage_tiles = 0;
+    unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
     hipStream_t last_stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;  // pending timed runs
@@ -172,17 +173,26 @@ namespace {
 
 int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles);
 
+uint32_t policy_of(Kind k) { return k == KIND_MERGE ? POLICY_MERGE : POLICY_SEARCH; }
+uint32_t units_per_tile(Kind k) { return k == KIND_MERGE ? (uint32_t)kU : (uint32_t)kUS; }
+uint32_t pad_units(Kind k) { return k == KIND_MERGE ? kC0 : kC0S; }
 // Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
-// can produce ((U+1)/2 + 1).
-constexpr uint32_t kStageHalf = kU / 2 + 64;
-constexpr uint32_t kStageWords = 2 * kStageHalf;
+// can produce (merge: (U+1)/2 + 1; search: U).
+uint32_t stage_words(Kind k, bool bsr)
+{
+    const uint32_t half = (k == KIND_MERGE ? (uint32_t)kU / 2 : (uint32_t)kUS) + 64;
+    return bsr ? 2 * half : half;
+}
+// The plan's unit prefix has a closed form only for merge-policy CSR batches (and single pairs).
+bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gather && h->kind == KIND_MERGE); }
 
 // Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields.
 int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
 {
-    const uint64_t ntiles = std::max<uint64_t>(1, (utotal + kU - 1) / kU);
+    const uint64_t U = units_per_tile(h->kind);
+    const uint64_t ntiles = std::max<uint64_t>(1, (utotal + U - 1) / U);
     if (ntiles >= 0x7FFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu tiles", (unsigned long long)ntiles);
-    if (P > 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
+    if (P >= 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
     if (!h->ws || P > h->cap_pairs || ntiles > h->cap_tiles) {
         if (h->ws) {
             if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
@@ -197,11 +207,9 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
     }
     h->plan.utotal = utotal;
     h->plan.ntiles = (uint32_t)ntiles;
-    h->plan.c0 = kC0;
-    h->plan.U = kU;
-    h->plan.stage_words = kStageWords;
-    const uint64_t gp = std::min<uint64_t>((P + 1 + 255) / 256, 8192);
-    h->grid_plan = (int)std::max<uint64_t>(gp, 1);
+    h->plan.c0 = pad_units(h->kind);
+    h->plan.U = (uint32_t)U;
+    h->plan.policy = policy_of(h->kind);
     return SETOPS_OK;
 }
 
@@ -213,20 +221,22 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
         off = align_up(off + bytes, 256);
         return o;
     };
+    const uint32_t sw = stage_words(h->kind, h->bsr);
     const size_t o_uoff = take((P + 1) * 8);
     const size_t o_tf = take(ntiles * 4);
     const size_t o_tl = take(ntiles * 4);
+    const size_t o_sa = take(ntiles * 4);
+    const size_t o_sb = take(ntiles * 4);
     const size_t o_ga = take((ntiles + 1) * 8);
     const size_t o_gb = take((ntiles + 1) * 8);
     const size_t o_tc = take(ntiles * 4);
     const size_t o_te = take(ntiles * 4);
     const size_t o_to = take((ntiles + 1) * 8);
-    const size_t ngroups = (ntiles + kScanGroup - 1) / kScanGroup;
+    const size_t ngroups = (std::max<uint64_t>(ntiles, P) + kScanGroup - 1) / kScanGroup;
     const size_t o_gc = take(ngroups * 8);
     const size_t o_ge = take(ngroups * 8);
     const size_t o_pl = take(P * 4 + 4);
-    const size_t o_misc = take(256);  // error word, single-pair offsets
-    const size_t o_stage = take(ntiles * (size_t)kStageWords * 4);
+    const size_t o_misc = take(256);  // error word, scratch totals, single-pair offsets
     void *ws = nullptr;
     if (hipMalloc(&ws, off) != hipSuccess) {
         (void)hipGetLastError();
@@ -238,27 +248,120 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
     h->plan.uoff = reinterpret_cast<uint64_t *>(b + o_uoff);
     h->plan.tile_first = reinterpret_cast<uint32_t *>(b + o_tf);
     h->plan.tile_last = reinterpret_cast<uint32_t *>(b + o_tl);
+    h->plan.tile_sa = reinterpret_cast<uint32_t *>(b + o_sa);
+    h->plan.tile_sb = reinterpret_cast<uint32_t *>(b + o_sb);
     h->plan.tile_ga = reinterpret_cast<uint64_t *>(b + o_ga);
     h->plan.tile_gb = reinterpret_cast<uint64_t *>(b + o_gb);
     h->plan.tile_count = reinterpret_cast<uint32_t *>(b + o_tc);
     h->plan.tile_elems = reinterpret_cast<uint32_t *>(b + o_te);
     h->plan.tile_off = reinterpret_cast<uint64_t *>(b + o_to);
-    h->plan.group_cnt = reinterpret_cast<unsigned long long *>(b + o_gc);
-    h->plan.group_el = reinterpret_cast<unsigned long long *>(b + o_ge);
+    h->plan.gsum = reinterpret_cast<unsigned long long *>(b + o_gc);
+    h->plan.gsum2 = reinterpret_cast<unsigned long long *>(b + o_ge);
     h->plan.pair_local = reinterpret_cast<uint32_t *>(b + o_pl);
     h->d_error = reinterpret_cast<uint32_t *>(b + o_misc);
-    h->plan.single_off = reinterpret_cast<uint64_t *>(b + o_misc + 16);
-    h->plan.stage = reinterpret_cast<uint32_t *>(b + o_stage);
+    h->d_tot = reinterpret_cast<unsigned long long *>(b + o_misc + 16);
+    h->plan.single_off = reinterpret_cast<uint64_t *>(b + o_misc + 64);
+    h->plan.stage_words = sw;
+    return SETOPS_OK;
+}
+
+// Per-tile result staging, allocated on the first run that writes results (count-only runs
+// never need it), grow-only.
+int stage_reserve(setops_hip_batch *h)
+{
+    const uint64_t nt = h->plan.ntiles;
+    if (h->stage && h->stage_tiles >= nt) {
+        h->plan.stage = h->stage;
+        return SETOPS_OK;
+    }
+    if (h->stage) {
+        if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+        HIP_TRY(hipFree(h->stage));
+        h->stage = nullptr;
+        h->stage_tiles = 0;
+    }
+    const size_t bytes = (size_t)nt * h->plan.stage_words * 4;
+    if (hipMalloc(&h->stage, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        h->stage = nullptr;
+        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %zu staging bytes failed", bytes);
+    }
+    h->stage_tiles = nt;
+    h->plan.stage = h->stage;
     return SETOPS_OK;
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-template <bool BSR, bool CO>
-void launch_merge(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+uint32_t grid_for(uint64_t n, uint64_t per_block)
 {
-    hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO>), dim3(h->plan.ntiles), dim3(kNT), 0, stream, h->in,
-                       h->plan, out);
+    const uint64_t g = (n + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, kMaxGrid));
+}
+
+// Unit prefix by the three-kernel scan (gather batches, search policy).
+int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
+    hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                       h->plan.gsum2, (uint64_t)ng, 0, h->d_tot);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((units_scan_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan, out);
+    HIP_TRY(hipGetLastError());
+    return SETOPS_OK;
+}
+
+// Total units of a batch whose prefix has no closed form: group sums and their scan on the
+// device, total read back (a synchronisation at handle creation only).
+int device_unit_total(setops_hip_batch *h, uint64_t *utotal)
+{
+    const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
+    hipStream_t s = g_ctx.stream;
+    hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, s, h->in, h->plan);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, s, h->plan.gsum, h->plan.gsum2,
+                       (uint64_t)ng, 0, h->d_tot);
+    HIP_TRY(hipGetLastError());
+    unsigned long long tot = 0;
+    HIP_TRY(hipMemcpyAsync(&tot, h->d_tot, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *utotal = tot;
+    return SETOPS_OK;
+}
+
+// Tile kernels run one workgroup per tile; batches of more than kMaxGrid tiles take several
+// launches (tile offset tile0).
+template <bool BSR, bool CO, bool GA>
+void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
+        hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO, GA>),
+                           dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
+                           h->in, h->plan, out, (uint32_t)t0);
+}
+
+template <bool BSR, bool CO, bool GALLOP>
+void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
+        hipLaunchKernelGGL((search_tile_kernel<kNT, kKS, kMaxSeg, kCapB, BSR, CO, GALLOP>),
+                           dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
+                           h->in, h->plan, out, (uint32_t)t0);
+}
+
+template <bool BSR, bool CO>
+void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    switch (h->kind) {
+    case KIND_MERGE:
+        if (h->gather) launch_merge_t<BSR, CO, true>(h, out, stream);
+        else launch_merge_t<BSR, CO, false>(h, out, stream);
+        break;
+    case KIND_GALLOP: launch_search_t<BSR, CO, true>(h, out, stream); break;
+    case KIND_BINARY: launch_search_t<BSR, CO, false>(h, out, stream); break;
+    }
 }
 
 int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64_t cap, uint64_t *d_out_off,
@@ -277,10 +380,19 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.counts = d_counts;
     out.totals = d_totals;
     out.error = h->d_error;
-    // plan: sizes only
-    hipLaunchKernelGGL(plan_csr_kernel, dim3(h->grid_plan), dim3(256), 0, stream, h->in, h->plan, out);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(plan_split_kernel, dim3((h->plan.ntiles + 1 + 255) / 256), dim3(256), 0, stream, h->in, h->plan);
+    if (!count_only) {
+        int rc = stage_reserve(h);
+        if (rc != SETOPS_OK) return rc;
+    }
+    // plan: unit prefix and tile maps, then tile starts
+    if (closed_form(h)) {
+        hipLaunchKernelGGL(plan_csr_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
+        HIP_TRY(hipGetLastError());
+    } else {
+        int rc = launch_units_scan(h, out, stream);
+        if (rc != SETOPS_OK) return rc;
+    }
+    hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in, h->plan);
     HIP_TRY(hipGetLastError());
     // count (and stage)
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
@@ -295,27 +407,29 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
         HIP_TRY(hipEventRecord(ev.first, stream));
     }
     if (h->bsr) {
-        if (count_only) launch_merge<true, true>(h, out, stream);
-        else launch_merge<true, false>(h, out, stream);
+        if (count_only) launch_tiles_b<true, true>(h, out, stream);
+        else launch_tiles_b<true, false>(h, out, stream);
     } else {
-        if (count_only) launch_merge<false, true>(h, out, stream);
-        else launch_merge<false, false>(h, out, stream);
+        if (count_only) launch_tiles_b<false, true>(h, out, stream);
+        else launch_tiles_b<false, false>(h, out, stream);
     }
     HIP_TRY(hipGetLastError());
     if (h->timing) {
         HIP_TRY(hipEventRecord(ev.second, stream));
         h->events.push_back(ev);
     }
-    // scan
+    // scan: group sums, one-block scan of the group sums (writes the totals), per-group scan
     const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
     hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3((h->plan.ntiles + kScanGroup - 1) / kScanGroup),
-                       dim3(kScanGroup), 0, stream, h->plan, out);
+    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                       h->plan.gsum2, (uint64_t)ngroups, 1, d_totals ? d_totals : h->d_tot);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, out);
     HIP_TRY(hipGetLastError());
     // write
     if (!count_only || d_out_off) {
-        const uint32_t gp = (h->plan.ntiles + 3) / 4;  // 4 tiles (waves) per workgroup
+        const uint32_t gp = grid_for(h->plan.ntiles, 4);  // 4 tiles (waves) per workgroup
         if (h->bsr)
             hipLaunchKernelGGL((place_tiles_kernel<256, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
         else
@@ -326,16 +440,18 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     return SETOPS_OK;
 }
 
+// Shared by the CSR and gather constructors: a_end == NULL means CSR (a_end = a_beg + 1).
 int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const uint32_t *d_as, uint64_t a_len,
-                 const uint64_t *d_a_off, const uint32_t *d_b, const uint32_t *d_bs, uint64_t b_len,
-                 const uint64_t *d_b_off, uint64_t npairs)
+                 const uint64_t *d_a_beg, const uint64_t *d_a_end, const uint32_t *d_b, const uint32_t *d_bs,
+                 uint64_t b_len, const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs)
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
     *handle = nullptr;
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
-    if (!d_a_off || !d_b_off) return fail(SETOPS_ERR_INVALID, "offset arrays must not be NULL");
+    const bool gather = d_a_end != nullptr;
+    if (!d_a_beg || !d_b_beg || (gather && !d_b_end)) return fail(SETOPS_ERR_INVALID, "offset arrays must not be NULL");
     if ((a_len && !d_a) || (b_len && !d_b)) return fail(SETOPS_ERR_INVALID, "set pools must not be NULL");
     if (!aligned16(d_a) || !aligned16(d_b)) return fail(SETOPS_ERR_INVALID, "set pools must be 16-byte aligned");
     if (e->bsr) {
@@ -346,21 +462,35 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     }
     auto *h = new setops_hip_batch();
     h->algo = algo;
+    h->kind = e->kind;
     h->bsr = e->bsr;
+    h->gather = gather;
     h->P = npairs;
     h->in.A = d_a;
     h->in.As = d_as;
     h->in.B = d_b;
     h->in.Bs = d_bs;
-    h->in.a_off = d_a_off;
-    h->in.b_off = d_b_off;
+    h->in.a_beg = d_a_beg;
+    h->in.a_end = gather ? d_a_end : d_a_beg + 1;
+    h->in.b_beg = d_b_beg;
+    h->in.b_end = gather ? d_b_end : d_b_beg + 1;
     h->in.a_pool = a_len;
     h->in.b_pool = b_len;
     h->in.P = npairs;
-    const uint64_t utotal = a_len + b_len + (uint64_t)kC0 * npairs;
+    h->in.contig = gather ? 0u : 1u;
     h->plan.single = 0;
-    int rc = batch_reserve(h, npairs, utotal);
+    int rc;
+    if (!gather && e->kind == KIND_MERGE) {
+        rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)kC0 * npairs);
+    } else {
+        // size the scan arrays first, then count units on the device
+        rc = batch_reserve(h, npairs, 1);
+        uint64_t utotal = 0;
+        if (rc == SETOPS_OK) rc = device_unit_total(h, &utotal);
+        if (rc == SETOPS_OK) rc = batch_reserve(h, npairs, utotal);
+    }
     if (rc != SETOPS_OK) {
+        if (h->ws) (void)hipFree(h->ws);
         delete h;
         return rc;
     }
@@ -381,6 +511,7 @@ void batch_free(setops_hip_batch *h)
         (void)hipEventDestroy(e.second);
     }
     if (h->ws) (void)hipFree(h->ws);
+    if (h->stage) (void)hipFree(h->stage);
     delete h;
 }
 
@@ -405,34 +536,38 @@ int host_scratch(size_t bytes, void **p)
     return SETOPS_OK;
 }
 
-// Context-cached handle for single pairs (plain / BSR).  Its offsets live in the workspace and
-// are written by the plan kernel, so a single-pair run needs no host-to-device copy of sizes.
+// Context-cached handle for single pairs, one per algorithm.  Its offsets live in the workspace
+// and are written by the plan kernel, so a single-pair run needs no host-to-device copy of sizes.
 int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, const uint32_t *dB,
                   const uint32_t *dBs, size_t nb, uint32_t *dO, uint32_t *dOs, size_t cap, unsigned long long *dTot,
                   hipStream_t s)
 {
     const AlgoEntry *e = algo_entry(algo);
-    setops_hip_batch *&h = g_ctx.single[e->bsr ? 1 : 0];
+    setops_hip_batch *&h = g_ctx.single[algo];
     if (!h) {
         h = new setops_hip_batch();
+        h->algo = algo;
+        h->kind = e->kind;
         h->bsr = e->bsr;
     }
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
-    h->algo = algo;
     h->P = 1;
-    const uint64_t utotal = (uint64_t)na + nb + kC0;
-    int rc = batch_reserve(h, 1, utotal);
+    const uint64_t units = (e->kind == KIND_MERGE ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind);
+    h->plan.single = 1;
+    int rc = batch_reserve(h, 1, units);
     if (rc != SETOPS_OK) return rc;
     h->in.A = dA;
     h->in.As = dAs;
     h->in.B = dB;
     h->in.Bs = dBs;
-    h->in.a_off = h->plan.single_off;
-    h->in.b_off = h->plan.single_off + 2;
+    h->in.a_beg = h->plan.single_off;
+    h->in.a_end = h->plan.single_off + 1;
+    h->in.b_beg = h->plan.single_off + 2;
+    h->in.b_end = h->plan.single_off + 3;
     h->in.a_pool = na;
     h->in.b_pool = nb;
     h->in.P = 1;
-    h->plan.single = 1;
+    h->in.contig = 1;
     h->plan.single_na = na;
     h->plan.single_nb = nb;
     return batch_launch(h, dO, dOs, cap, nullptr, nullptr, dTot, s);
@@ -626,7 +761,19 @@ int setops_hip_batch_create(setops_hip_batch **handle, int algo, const uint32_t 
                             uint64_t b_len, const uint64_t *d_b_off, uint64_t npairs)
 {
     ENSURE_INIT();
-    return batch_create(handle, algo, d_a, d_a_states, a_len, d_a_off, d_b, d_b_states, b_len, d_b_off, npairs);
+    return batch_create(handle, algo, d_a, d_a_states, a_len, d_a_off, nullptr, d_b, d_b_states, b_len, d_b_off,
+                        nullptr, npairs);
+}
+
+int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const uint32_t *d_a,
+                                   const uint32_t *d_a_states, uint64_t a_len, const uint64_t *d_a_beg,
+                                   const uint64_t *d_a_end, const uint32_t *d_b, const uint32_t *d_b_states,
+                                   uint64_t b_len, const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs)
+{
+    ENSURE_INIT();
+    if (!d_a_end || !d_b_end) return fail(SETOPS_ERR_INVALID, "end arrays must not be NULL");
+    return batch_create(handle, algo, d_a, d_a_states, a_len, d_a_beg, d_a_end, d_b, d_b_states, b_len, d_b_beg,
+                        d_b_end, npairs);
 }
 
 int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_out_states, uint64_t out_cap,
@@ -677,7 +824,7 @@ int setops_hip_batch_info(setops_hip_batch *handle, uint64_t *ntiles, uint64_t *
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
     if (ntiles) *ntiles = handle->plan.ntiles;
-    if (grid) *grid = (uint64_t)handle->plan.ntiles;
+    if (grid) *grid = handle->plan.ntiles;
     if (units) *units = handle->plan.utotal;
     return SETOPS_OK;
 }

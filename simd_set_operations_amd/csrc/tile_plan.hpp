@@ -1,0 +1,481 @@
+// tile_plan.hpp -- batch description, unit-space planning, tile scan and result placement (gfx950).
+//
+// A batch is P pairs.  Pair p is A[a_beg[p] .. a_end[p]) against B[b_beg[p] .. b_end[p]) in two
+// element pools.  CSR batches (consecutive pairs adjacent in both pools) pass a_end = a_beg + 1;
+// "contig" marks them and lets a tile stage its whole A range and B range as two contiguous runs.
+// Gather batches (pairs anywhere in the pools: graph edges, corpus lists, intermediate results)
+// stage each pair's window separately.
+//
+// Work model.  Pair p owns units [uoff[p], uoff[p+1]) of one global unit space:
+//   merge policy:  |A| + |B| + c0 units (one per merge-path step, plus padding),
+//   search policy: min(|A|, |B|) + c0 units (one per probe of the smaller side, plus padding).
+// The c0 padding bounds how many pairs one tile can touch (<= U/c0 + 2), which sizes the LDS
+// segment table.  Tile t covers units [t*U, (t+1)*U): tiny pairs are packed many to a tile, large
+// pairs are split across tiles, every tile carries the same work.
+//
+// Pipeline (count -> scan -> write), no communication between workgroups:
+//   plan     unit prefix (closed form for CSR, a three-kernel scan for gather), tile -> pair maps
+//   split    where each tile starts inside its first pair (merge-path split or probe index)
+//   tile     one workgroup per tile: count and stage the tile's results (merge_tile / search_tile)
+//   scan     group sums -> one-block scan of the group sums -> per-group scan of tile counts
+//   place    move staged results to their final positions, per-pair output offsets
+#pragma once
+
+#include "tile_common.hpp"
+
+namespace setops {
+
+constexpr uint32_t kScanGroup = 1024;  // elements per group in the three-kernel scans
+
+struct PairBatch {
+    const uint32_t *A;   // A keys (values, or BSR bases)
+    const uint32_t *As;  // A BSR states (null for plain sets)
+    const uint32_t *B;
+    const uint32_t *Bs;
+    const uint64_t *a_beg;  // [P] first A element of pair p
+    const uint64_t *a_end;  // [P] one past the last A element of pair p
+    const uint64_t *b_beg;
+    const uint64_t *b_end;
+    uint64_t a_pool;  // elements readable at A (loads never go past)
+    uint64_t b_pool;
+    uint64_t P;
+    uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
+};
+
+enum : uint32_t { POLICY_MERGE = 0, POLICY_SEARCH = 1 };
+
+struct TilePlan {
+    uint64_t *uoff;        // [P+1] unit prefix
+    uint32_t *tile_first;  // [ntiles] pair containing the tile's first unit
+    uint32_t *tile_last;   // [ntiles] pair containing the tile's last unit
+    uint32_t *tile_sa;     // [ntiles] pair-relative A index where the tile starts (probe index: search)
+    uint32_t *tile_sb;     // [ntiles] pair-relative B index where the tile starts (merge only)
+    uint64_t *tile_ga;     // [ntiles+1] pool index of tile t's first A element (merge, CSR batches)
+    uint64_t *tile_gb;     // [ntiles+1] pool index of tile t's first B element (merge, CSR batches)
+    uint32_t *tile_count;  // [ntiles] result words of tile t
+    uint32_t *tile_elems;  // [ntiles] result elements of tile t (popcount for BSR)
+    uint64_t *tile_off;    // [ntiles+1] exclusive scan of tile_count
+    unsigned long long *gsum;   // [ngroups] group sums (pair units, then tile counts)
+    unsigned long long *gsum2;  // [ngroups] second group sums (tile elements)
+    uint32_t *stage;       // [ntiles * stage_words] per-tile staged outputs (keys, then BSR states)
+    uint32_t *pair_local;  // [P] offset of a pair's first result inside the tile where it starts
+    uint64_t utotal;
+    uint32_t ntiles;
+    uint32_t c0;           // padding units per pair
+    uint32_t U;            // units per tile
+    uint32_t stage_words;  // words per tile in `stage` (keys; states follow at +stage_words/2)
+    uint32_t policy;       // POLICY_MERGE or POLICY_SEARCH (how units are counted)
+    // single-pair mode: the plan kernel writes {0, na, 0, nb} to single_off, which serves as
+    // a_beg / a_end / b_beg / b_end
+    uint32_t single;
+    uint64_t *single_off;
+    uint64_t single_na, single_nb;
+};
+
+struct TileOut {
+    uint32_t *out;          // compacted keys (null in count-only mode)
+    uint32_t *out_s;        // compacted BSR states
+    uint64_t cap;           // capacity of out / out_s in elements
+    uint64_t *out_off;      // [P+1] output offset of each pair (may be null)
+    uint32_t *counts;       // [P] per-pair output count (may be null)
+    unsigned long long *totals;  // [2]: output words, output elements (popcount for BSR)
+    uint32_t *error;        // bit 0: output capacity exceeded, bit 1: tile bound violated
+};
+
+__device__ __forceinline__ uint64_t pair_units(const PairBatch &in, uint32_t policy, uint32_t c0, uint64_t p)
+{
+    const uint64_t na = in.a_end[p] - in.a_beg[p], nb = in.b_end[p] - in.b_beg[p];
+    return (policy == POLICY_SEARCH ? (na < nb ? na : nb) : na + nb) + c0;
+}
+
+// Records, for the pair owning units [u, un), the tiles it starts and ends.
+__device__ __forceinline__ void map_tiles(const TilePlan &plan, uint64_t p, uint64_t u, uint64_t un)
+{
+    const uint64_t U = plan.U;
+    for (uint64_t t = (u + U - 1) / U; t * U < un && t < plan.ntiles; t++) plan.tile_first[t] = (uint32_t)p;
+    for (uint64_t t = u / U; t < plan.ntiles; t++) {
+        uint64_t last = (t + 1) * U;
+        if (last > plan.utotal) last = plan.utotal;
+        last -= 1;
+        if (last >= un) break;
+        if (last >= u) plan.tile_last[t] = (uint32_t)p;
+    }
+}
+
+__device__ __forceinline__ void plan_reset(const TileOut &out)
+{
+    if (out.totals) {
+        out.totals[0] = 0;
+        out.totals[1] = 0;
+    }
+    if (out.error) *out.error = 0;
+}
+
+// CSR batches: unit prefix in closed form, tile maps, zeroed per-pair counts.  Grid-stride over
+// pairs.  In single-pair mode it also writes the pair's offsets.
+__global__ void plan_csr_kernel(PairBatch in, TilePlan plan, TileOut out)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    auto abeg = [&](uint64_t p) -> uint64_t { return plan.single ? 0 : in.a_beg[p]; };
+    auto bbeg = [&](uint64_t p) -> uint64_t { return plan.single ? 0 : in.b_beg[p]; };
+    auto aend = [&](uint64_t p) -> uint64_t { return plan.single ? plan.single_na : in.a_end[p]; };
+    auto bend = [&](uint64_t p) -> uint64_t { return plan.single ? plan.single_nb : in.b_end[p]; };
+    if (plan.single && gid == 0) {
+        plan.single_off[0] = 0;
+        plan.single_off[1] = plan.single_na;
+        plan.single_off[2] = 0;
+        plan.single_off[3] = plan.single_nb;
+    }
+    const uint64_t a0 = abeg(0), b0 = bbeg(0);
+    for (uint64_t p = gid; p < in.P; p += gsz) {
+        const uint64_t ab = abeg(p) - a0, bb = bbeg(p) - b0;
+        const uint64_t ae = aend(p) - a0, be = bend(p) - b0;
+        uint64_t u, un;
+        if (plan.policy == POLICY_SEARCH) {
+            // min() does not telescope: only single-pair plans take this branch
+            u = 0;
+            un = ((ae - ab) < (be - bb) ? (ae - ab) : (be - bb)) + plan.c0;
+        } else {
+            u = ab + bb + (uint64_t)plan.c0 * p;
+            un = ae + be + (uint64_t)plan.c0 * (p + 1);
+        }
+        plan.uoff[p] = u;
+        if (p + 1 == in.P) plan.uoff[in.P] = un;
+        map_tiles(plan, p, u, un);
+        if (out.counts) out.counts[p] = 0;
+    }
+    if (gid == 0) plan_reset(out);
+}
+
+// ---- three-kernel scan of pair units (gather batches and the search policy) -------------------
+
+// Group sums: one workgroup per kScanGroup pairs.
+template <int NT>
+__global__ __launch_bounds__(NT) void units_reduce_kernel(PairBatch in, TilePlan plan)
+{
+    static_assert(NT == (int)kScanGroup, "one thread per element of a group");
+    __shared__ unsigned long long red[NT / kWave];
+    const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
+    unsigned long long v = (p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v += __shfl_xor(v, d, kWave);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < NT / kWave; w++) s += red[w];
+        plan.gsum[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of n group sums in place by one workgroup (and of gsum2 when `two`); the
+// totals go to tot[0] (and tot[1]).
+template <int NT>
+__global__ __launch_bounds__(NT) void groups_scan_kernel(unsigned long long *gsum, unsigned long long *gsum2,
+                                                         uint64_t n, int two, unsigned long long *tot)
+{
+    __shared__ unsigned long long ws[NT / kWave][2];
+    unsigned long long carry = 0, carry2 = 0;
+    const uint32_t w = threadIdx.x / kWave, l = lane_id();
+    for (uint64_t base = 0; base < n; base += NT) {
+        const uint64_t i = base + threadIdx.x;
+        const unsigned long long v = (i < n) ? gsum[i] : 0ull;
+        const unsigned long long v2 = (two && i < n) ? gsum2[i] : 0ull;
+        unsigned long long x = v, x2 = v2;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const unsigned long long y = __shfl_up(x, d, kWave), y2 = __shfl_up(x2, d, kWave);
+            if (l >= (uint32_t)d) {
+                x += y;
+                x2 += y2;
+            }
+        }
+        if (l == kWave - 1) {
+            ws[w][0] = x;
+            ws[w][1] = x2;
+        }
+        __syncthreads();
+        unsigned long long pre = 0, pre2 = 0, all = 0, all2 = 0;
+        for (int k = 0; k < NT / kWave; k++) {
+            if (k < (int)w) {
+                pre += ws[k][0];
+                pre2 += ws[k][1];
+            }
+            all += ws[k][0];
+            all2 += ws[k][1];
+        }
+        if (i < n) {
+            gsum[i] = carry + pre + x - v;
+            if (two) gsum2[i] = carry2 + pre2 + x2 - v2;
+        }
+        carry += all;
+        carry2 += all2;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = carry;
+        if (two) tot[1] = carry2;
+    }
+}
+
+// Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.
+template <int NT>
+__global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out)
+{
+    __shared__ unsigned long long wsum[NT / kWave];
+    const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
+    const unsigned long long v = (p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
+    // 64-bit block scan: wave scan, then wave totals
+    unsigned long long x = v;
+    const uint32_t l = lane_id(), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, kWave);
+        if (l >= (uint32_t)d) x += y;
+    }
+    if (l == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    unsigned long long pre = plan.gsum[blockIdx.x];
+    for (int k = 0; k < (int)w; k++) pre += wsum[k];
+    if (p < in.P) {
+        const uint64_t u = pre + x - v, un = pre + x;
+        plan.uoff[p] = u;
+        if (p + 1 == in.P) plan.uoff[in.P] = un;
+        map_tiles(plan, p, u, un);
+        if (out.counts) out.counts[p] = 0;
+    }
+    if (p == 0) plan_reset(out);
+}
+
+// Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
+// A index i and B index d - i with every A[x < i] <= B[d-1-...]), or the probe index (search).
+__global__ void plan_split_kernel(PairBatch in, TilePlan plan)
+{
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < plan.ntiles; t += gsz) {
+        const uint64_t p = plan.tile_first[t];
+        const uint64_t a = in.a_beg[p], b = in.b_beg[p];
+        const uint64_t na = in.a_end[p] - a, nb = in.b_end[p] - b;
+        const uint64_t rel = (uint64_t)t * plan.U - plan.uoff[p];
+        if (plan.policy == POLICY_SEARCH) {
+            const uint64_t np = na < nb ? na : nb;
+            plan.tile_sa[t] = (uint32_t)(rel < np ? rel : np);
+            plan.tile_sb[t] = 0;
+            continue;
+        }
+        const uint64_t d = rel < na + nb ? rel : na + nb;
+        uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+        const uint32_t *__restrict__ A = in.A + a;
+        const uint32_t *__restrict__ B = in.B + b;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
+        }
+        plan.tile_sa[t] = (uint32_t)lo;
+        plan.tile_sb[t] = (uint32_t)(d - lo);
+        if (in.contig) {
+            // CSR: a tile's A run is [tile_ga[t], tile_ga[t+1]) -- one load for the staging loop
+            plan.tile_ga[t] = a + lo;
+            plan.tile_gb[t] = b + (d - lo);
+            if (t + 1 == plan.ntiles) {
+                plan.tile_ga[t + 1] = in.a_end[in.P - 1];
+                plan.tile_gb[t + 1] = in.b_end[in.P - 1];
+            }
+        }
+    }
+}
+
+// ---- tile scan ---------------------------------------------------------------------------------
+
+// Group sums of the tile counts (one workgroup per group of kScanGroup tiles, no atomics).
+template <int NT>
+__global__ __launch_bounds__(NT) void reduce_tiles_kernel(TilePlan plan)
+{
+    static_assert(NT == (int)kScanGroup, "one thread per tile of a group");
+    __shared__ unsigned long long red[NT / kWave][2];
+    const uint32_t i = blockIdx.x * kScanGroup + threadIdx.x;
+    unsigned long long c = 0, e = 0;
+    if (i < plan.ntiles) {
+        c = plan.tile_count[i];
+        e = plan.tile_elems[i];
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        c += __shfl_xor(c, d, kWave);
+        e += __shfl_xor(e, d, kWave);
+    }
+    if (lane_id() == 0) {
+        red[threadIdx.x / kWave][0] = c;
+        red[threadIdx.x / kWave][1] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long sc = 0, se = 0;
+        for (int w = 0; w < NT / kWave; w++) {
+            sc += red[w][0];
+            se += red[w][1];
+        }
+        plan.gsum[blockIdx.x] = sc;
+        plan.gsum2[blockIdx.x] = se;
+    }
+}
+
+// Per-group block scan of the tile counts plus the (already scanned) group base.  The last
+// group writes tile_off[ntiles] and flags a clipped output.  Totals were written by the group
+// scan.
+template <int NT>
+__global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut out)
+{
+    static_assert(NT == (int)kScanGroup, "one thread per tile of a group");
+    __shared__ uint32_t scratch[NT / kWave];
+    const uint32_t g = blockIdx.x;
+    const unsigned long long base = plan.gsum[g];
+    const uint32_t i = g * kScanGroup + threadIdx.x;
+    const uint32_t c = (i < plan.ntiles) ? plan.tile_count[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<NT>(c, scratch, &tot);
+    if (i < plan.ntiles) plan.tile_off[i] = base + ex;
+    if (i + 1 == plan.ntiles) {
+        const unsigned long long total = base + ex + c;
+        plan.tile_off[plan.ntiles] = total;
+        if (out.out && total > out.cap && out.error) atomicOr(out.error, 1u);
+    }
+}
+
+// Write: moves each tile's staged results to their final position and turns in-tile pair offsets
+// into output offsets.  One wave per tile, NT/64 tiles per workgroup, grid-stride.
+template <int NT, bool BSR>
+__global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out)
+{
+    const uint32_t l = lane_id();
+    const uint32_t wpb = NT / kWave;
+    for (uint64_t t = (uint64_t)blockIdx.x * wpb + threadIdx.x / kWave; t < plan.ntiles;
+         t += (uint64_t)gridDim.x * wpb) {
+        const uint64_t base = plan.tile_off[t];
+        const uint32_t n = plan.tile_count[t];
+        if (out.out) {
+            const uint32_t *sk = plan.stage + t * plan.stage_words;
+            const uint32_t *ss = sk + plan.stage_words / 2;
+            uint64_t lim = n;
+            if (base + n > out.cap) lim = (base < out.cap) ? out.cap - base : 0;
+            for (uint32_t x = l; x < lim; x += kWave) {
+                out.out[base + x] = sk[x];
+                if constexpr (BSR) out.out_s[base + x] = ss[x];
+            }
+        }
+        if (out.out_off) {
+            const uint64_t u0 = t * plan.U;
+            const uint32_t pf = plan.tile_first[t], pl = plan.tile_last[t];
+            for (uint32_t p = pf + l; p <= pl; p += kWave)
+                if (plan.uoff[p] >= u0) out.out_off[p] = base + plan.pair_local[p];
+            if (t == plan.ntiles - 1 && l == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
+        }
+    }
+}
+
+// ---- per-tile segment geometry -------------------------------------------------------------------
+
+// One covered pair of a tile: its element windows (pair-relative) and where the tile's units
+// reach it.  Computed by thread s for segment s.
+struct SegWin {
+    uint64_t p;
+    uint64_t a, b;     // pool index of the pair's first A / B element
+    uint64_t na, nb;   // pair sizes
+    uint64_t i0, i1;   // A window [i0, i1)
+    uint64_t j0, j1;   // B window [j0, j1) (merge), unused for search
+    uint32_t u;        // tile-relative unit where the segment starts
+    uint32_t flags;    // bit0: pair starts in this tile, bit1: pair ends in this tile
+};
+
+// Window of segment s of tile t (pairs p_first .. p_last); the last segment ends where the next
+// tile starts if that tile starts in the same pair, else at the pair's end.
+__device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan &plan, uint32_t t, uint32_t p_first,
+                                             uint32_t nseg, uint32_t s, uint32_t ulen)
+{
+    SegWin w;
+    const uint64_t p = (uint64_t)p_first + s;
+    w.p = p;
+    w.a = in.a_beg[p];
+    w.b = in.b_beg[p];
+    w.na = in.a_end[p] - w.a;
+    w.nb = in.b_end[p] - w.b;
+    const uint64_t up = plan.uoff[p], upn = plan.uoff[p + 1];
+    const uint64_t u0 = (uint64_t)t * plan.U;
+    w.i0 = (s == 0) ? plan.tile_sa[t] : 0;
+    w.j0 = (s == 0) ? plan.tile_sb[t] : 0;
+    const bool cont = (s == nseg - 1) && (t + 1 < plan.ntiles) && plan.tile_first[t + 1] == (uint32_t)p;
+    if (plan.policy == POLICY_SEARCH) {
+        const uint64_t np = w.na < w.nb ? w.na : w.nb;
+        w.i1 = cont ? plan.tile_sa[t + 1] : np;
+        w.j0 = 0;
+        w.j1 = 0;
+    } else {
+        w.i1 = cont ? plan.tile_sa[t + 1] : w.na;
+        w.j1 = cont ? plan.tile_sb[t + 1] : w.nb;
+    }
+    w.u = (up > u0) ? (uint32_t)(up - u0) : 0u;
+    w.flags = (up >= u0 ? 1u : 0u) | (upn <= u0 + ulen ? 2u : 0u);
+    return w;
+}
+
+// Copies `n` elements starting at global element index g0 of `src` (pool length `pool`) into LDS
+// at dst[shift ...], where shift = g0 & 3: 16-byte loads of the aligned chunks covering the range,
+// one chunk per thread per iteration (coalesced), the pool's last partial chunk element-wise.
+template <int NT>
+__device__ __forceinline__ void stage_range(const uint32_t *__restrict__ src, uint64_t pool, uint64_t g0, uint32_t n,
+                                            uint32_t *dst)
+{
+    const uint64_t base = g0 & ~3ull;
+    const uint32_t span = (uint32_t)(g0 - base) + n;
+    const uint32_t chunks = (span + 3) / 4;
+    for (uint32_t c = threadIdx.x; c < chunks; c += NT) {
+        const uint64_t e = base + 4ull * c;
+        uint4 v;
+        if (e + 4 <= pool) {
+            v = *reinterpret_cast<const uint4 *>(src + e);
+        } else {
+            v.x = (e + 0 < pool) ? src[e + 0] : 0u;
+            v.y = (e + 1 < pool) ? src[e + 1] : 0u;
+            v.z = (e + 2 < pool) ? src[e + 2] : 0u;
+            v.w = (e + 3 < pool) ? src[e + 3] : 0u;
+        }
+        *reinterpret_cast<uint4 *>(dst + 4 * c) = v;
+    }
+}
+
+// Gather staging: nwin windows, window w = src_w[g0_w .. g0_w + n_w), each placed in LDS at
+// dst + 4*cpfx[w] + (g0_w & 3) as whole aligned 16-byte chunks.  cpfx is the exclusive prefix of
+// the windows' chunk counts (cpfx[nwin] = total).  Each thread takes chunks c, c + NT, ... and
+// finds the owning window by binary search over cpfx.
+template <int NT>
+__device__ __forceinline__ void stage_windows(const uint32_t *__restrict__ srcA, uint64_t poolA,
+                                              const uint32_t *__restrict__ srcB, uint64_t poolB,
+                                              const uint32_t *cpfx, const uint64_t *gbase, const uint32_t *wsrc,
+                                              uint32_t nwin, uint32_t *dst)
+{
+    const uint32_t total = cpfx[nwin];
+    for (uint32_t c = threadIdx.x; c < total; c += NT) {
+        uint32_t lo = 0, hi = nwin;  // largest w with cpfx[w] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cpfx[mid] <= c) lo = mid; else hi = mid;
+        }
+        const bool fromB = wsrc[lo] != 0;
+        const uint32_t *__restrict__ src = fromB ? srcB : srcA;
+        const uint64_t pool = fromB ? poolB : poolA;
+        const uint64_t e = gbase[lo] + 4ull * (c - cpfx[lo]);
+        uint4 v;
+        if (e + 4 <= pool) {
+            v = *reinterpret_cast<const uint4 *>(src + e);
+        } else {
+            v.x = (e + 0 < pool) ? src[e + 0] : 0u;
+            v.y = (e + 1 < pool) ? src[e + 1] : 0u;
+            v.z = (e + 2 < pool) ? src[e + 2] : 0u;
+            v.w = (e + 3 < pool) ? src[e + 3] : 0u;
+        }
+        *reinterpret_cast<uint4 *>(dst + 4 * c) = v;
+    }
+}
+
+} // namespace setops

@@ -8,7 +8,11 @@ The workloads are the benchmark configurations of this path:
   with r log-uniform in [1, 1024]; element space = 10 x larger; shared = round(sel x smaller),
   sel ~ U[0,1].
 
-Both follow the reference generator's planted-shared-then-disjoint semantics
+* ``skewed``       -- one resident large set of 2^24 over {0..2^28-1} against 4,096 small sets of
+  2^8, each with round(0.1 x 256) = 26 elements drawn from the large set and the rest from its
+  complement.
+
+All follow the reference generator's planted-shared-then-disjoint semantics
 (benchmark/src/generators.rs:30-110).
 """
 import ctypes
@@ -20,6 +24,7 @@ from . import _native
 _c = ctypes
 
 PLAIN_PAIR = dict(universe=1 << 20, size=1 << 16, shared=19661, seed=1001)
+SKEWED = dict(universe=1 << 28, large=1 << 24, nsmall=4096, small=256, shared=26, seed=1003)
 UNEVEN = dict(npairs=65536, lmin=6.0, lmax=14.0, log2_skew=10.0, density=0.1, seed=1002)
 
 
@@ -70,3 +75,15 @@ def uneven_batch(npairs=UNEVEN["npairs"], seed=UNEVEN["seed"], lmin=UNEVEN["lmin
     if with_shared:
         return A, a_off, B, b_off, shared
     return A, a_off, B, b_off
+
+
+def skewed(seed=SKEWED["seed"], universe=SKEWED["universe"], large=SKEWED["large"], nsmall=SKEWED["nsmall"],
+           small=SKEWED["small"], shared=SKEWED["shared"]):
+    """Skewed batch: (large[large], smalls[nsmall, small]) -- every small set has exactly `shared`
+    elements of the large set."""
+    L = np.zeros(large, np.uint32)
+    S = np.zeros((nsmall, small), np.uint32)
+    rc = _native.datagen().setops_gen_skewed(seed, universe, large, nsmall, small, shared, _p(L), _p(S))
+    if rc != 0:
+        raise ValueError("skewed workload does not fit its element space")
+    return L, S

@@ -14,8 +14,8 @@ from simd_set_operations_amd.visitor import Counter, VecWriter
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
-PLAIN = ["hip_branchless_merge"]
-BSR = ["hip_branchless_merge_bsr"]
+PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search"]
+BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr"]
 
 
 def announce(msg):
@@ -187,7 +187,8 @@ def _csr(sets):
     return data, off
 
 
-def test_batch_mixed_sizes():
+@pytest.mark.parametrize("name", PLAIN)
+def test_batch_mixed_sizes(name):
     from simd_set_operations_amd.batch import DeviceBatch
 
     rng = np.random.Generator(np.random.PCG64(61))
@@ -208,10 +209,10 @@ def test_batch_mixed_sizes():
         Bs.append(sorted_set(rng, lb, u))
     A, a_off = _csr(As)
     B, b_off = _csr(Bs)
-    announce(f"batch mixed P={P} |A|={A.size} |B|={B.size}")
+    announce(f"{name} batch mixed P={P} |A|={A.size} |B|={B.size}")
     counts, off, data = _batch_expect(A, a_off, B, b_off)
-    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
-    for rep in range(3):  # repeated runs reuse the handle (look-back words are reset each run)
+    db = DeviceBatch(name, A, a_off, B, b_off)
+    for rep in range(3):  # repeated runs reuse the handle and its plan
         db.run()
         g_off, g_cnt, g_data = db.results()
         assert np.array_equal(g_cnt, counts), rep
@@ -225,7 +226,8 @@ def test_batch_mixed_sizes():
     db.close()
 
 
-def test_batch_bsr():
+@pytest.mark.parametrize("name", BSR)
+def test_batch_bsr(name):
     from simd_set_operations_amd.batch import DeviceBatch
 
     rng = np.random.Generator(np.random.PCG64(71))
@@ -243,8 +245,8 @@ def test_batch_bsr():
     As_, _ = _csr(Ass)
     B, b_off = _csr(Bs)
     Bs_, _ = _csr(Bss)
-    announce(f"batch bsr P={P}")
-    db = DeviceBatch("hip_branchless_merge_bsr", A, a_off, B, b_off, A_states=As_, B_states=Bs_)
+    announce(f"{name} batch bsr P={P}")
+    db = DeviceBatch(name, A, a_off, B, b_off, A_states=As_, B_states=Bs_)
     db.run()
     g_off, g_cnt, g_b, g_s = db.results()
     exp_cnt = np.array([x.size for x in exp_b], np.uint32)
@@ -258,15 +260,16 @@ def test_batch_bsr():
     db.close()
 
 
-def test_uneven_batch_full_size():
+@pytest.mark.parametrize("name", ["hip_branchless_merge", "hip_galloping"])
+def test_uneven_batch_full_size(name):
     """The 65,536-pair uneven workload: per-pair counts equal the planted shared counts (a
     size-independent property of the generator), every output run is strictly increasing, and
     sampled pairs match the oracle exactly."""
     from simd_set_operations_amd.batch import DeviceBatch
 
     A, a_off, B, b_off, shared = datagen.uneven_batch(with_shared=True)
-    announce(f"uneven batch P={a_off.size - 1} E={A.size + B.size}")
-    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
+    announce(f"{name} uneven batch P={a_off.size - 1} E={A.size + B.size}")
+    db = DeviceBatch(name, A, a_off, B, b_off)
     db.run()
     g_off, g_cnt, g_data = db.results()
     assert np.array_equal(g_cnt, shared)
@@ -281,3 +284,95 @@ def test_uneven_batch_full_size():
         exp = oracle.branchless_merge(A[a_off[p]:a_off[p + 1]], B[b_off[p]:b_off[p + 1]])
         assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), p
     db.close()
+
+
+# ---- gather batches -----------------------------------------------------------------------------
+def _gather_case(rng, nsets, P, max_len):
+    """A pool of sorted sets and P pairs drawn from it (pairs share and repeat sets)."""
+    sets = []
+    for i in range(nsets):
+        ln = int(rng.integers(0, max_len)) if i % 5 else int(rng.integers(0, 8))
+        sets.append(sorted_set(rng, ln, max(4 * ln, 16)))
+    pool, off = _csr(sets)
+    ia = rng.integers(0, nsets, size=P)
+    ib = rng.integers(0, nsets, size=P)
+    return sets, pool, off, ia, ib
+
+
+@pytest.mark.parametrize("name", PLAIN)
+def test_gather_batch(name):
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(91))
+    sets, pool, off, ia, ib = _gather_case(rng, 400, 5000, 3000)
+    # a second pool so A and B come from different allocations; B sets are sub-ranges too
+    announce(f"{name} gather P={ia.size} pool={pool.size}")
+    a_beg, a_end = off[ia], off[ia + 1]
+    b_beg = off[ib] + (np.diff(off)[ib] // 4)  # suffix of each set: still sorted
+    b_end = off[ib + 1]
+    gb = GatherBatch(name, pool, a_beg, a_end, pool, b_beg, b_end)
+    for rep in range(2):
+        gb.run()
+        g_off, g_cnt, g_data = gb.results()
+        for p in range(ia.size):
+            exp = oracle.branchless_merge(pool[a_beg[p]:a_end[p]], pool[b_beg[p]:b_end[p]])
+            assert g_cnt[p] == exp.size, (rep, p)
+            assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), (rep, p)
+    gb.run(count_only=True)
+    words, _ = gb.totals()
+    assert words == int(g_cnt.sum())
+    gb.close()
+
+
+@pytest.mark.parametrize("name", BSR)
+def test_gather_batch_bsr(name):
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(93))
+    sets = [sorted_set(rng, int(rng.integers(0, 5000)), int(rng.integers(64, 60000))) for _ in range(200)]
+    bsrs = [BsrVec.from_sorted(x) for x in sets]
+    bases, off = _csr([b.bases for b in bsrs])
+    states, _ = _csr([b.states for b in bsrs])
+    P = 3000
+    ia, ib = rng.integers(0, 200, size=P), rng.integers(0, 200, size=P)
+    announce(f"{name} gather bsr P={P}")
+    gb = GatherBatch(name, bases, off[ia], off[ia + 1], bases, off[ib], off[ib + 1], A_states=states,
+                     B_states=states)
+    gb.run()
+    g_off, g_cnt, g_b, g_s = gb.results()
+    for p in range(P):
+        eb, es = oracle.bsr_intersect("branchless_merge_bsr", bsrs[ia[p]].bases, bsrs[ia[p]].states,
+                                      bsrs[ib[p]].bases, bsrs[ib[p]].states)
+        assert np.array_equal(g_b[g_off[p]:g_off[p + 1]], eb), p
+        assert np.array_equal(g_s[g_off[p]:g_off[p + 1]], es), p
+    gb.close()
+
+
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_branchless_merge"])
+def test_skewed_shared_large_set(name):
+    """The skewed workload's shape at reduced size: one large set shared by every pair (searched
+    in global memory when its narrowed window does not fit LDS) against many small sets, each with
+    a planted share drawn from the large set."""
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(95))
+    large = sorted_set(rng, 1 << 20, 1 << 24)
+    smalls, shared = [], []
+    for i in range(512):
+        m = int(rng.integers(0, 300))
+        hit = rng.choice(large, size=m // 10, replace=False)
+        miss = np.setdiff1d(sorted_set(rng, m, 1 << 24), large)
+        s_ = np.union1d(hit, miss).astype(np.uint32)
+        smalls.append(s_)
+        shared.append(hit.size)
+    pool, off = _csr(smalls)
+    P = len(smalls)
+    announce(f"{name} skewed P={P} |L|={large.size}")
+    gb = GatherBatch(name, pool, off[:-1], off[1:], large, np.zeros(P, np.uint64),
+                     np.full(P, large.size, np.uint64))
+    gb.run()
+    g_off, g_cnt, g_data = gb.results()
+    assert np.array_equal(g_cnt, np.array(shared, np.uint32))
+    for p in range(P):
+        assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], np.intersect1d(smalls[p], large)), p
+    gb.close()
