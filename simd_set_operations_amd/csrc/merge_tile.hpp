@@ -20,7 +20,7 @@
 
 namespace setops {
 
-template <int NT, int K, int MAXSEG, bool BSR, bool GATHER>
+template <int NT, int K, int MAXSEG, bool BSR, bool GATHER, bool BLOCKP>
 struct MergeSmem {
     static constexpr int U = NT * K;
     // CSR: the A run and the B run (+ alignment shifts); gather: every window rounded out to
@@ -37,6 +37,7 @@ struct MergeSmem {
     uint32_t seg_flags[MAXSEG];  // bit0: pair starts in this tile, bit1: pair ends in this tile
     uint32_t lane_mask[NT];      // per-lane step mask of matches
     uint32_t lane_pfx[NT];       // per-lane exclusive prefix of matches
+    uint32_t lane_nfirst[BLOCKP ? NT : 1];  // block policies: lane's A elements before a pair boundary
     uint32_t scan_scratch[NT / kWave];
     uint32_t wave_el[NT / kWave];
     // gather staging: chunk prefix, aligned global start and source pool of each window
@@ -45,16 +46,96 @@ struct MergeSmem {
     uint32_t win_src[GATHER ? 2 * MAXSEG : 1];
 };
 
+// Lane policies: how a lane intersects its slice of the tile.
+enum : int {
+    LP_MERGE = 0,    // K branch-free merge steps (merge.rs:39-87)
+    LP_SHUFFLE = 1,  // 4x4 all-pairs block compares (shuffling.rs:26-168, 174-356)
+    LP_QFILTER = 2,  // 4x4 blocks with a low-byte prefilter (qfilter.rs:32-180)
+};
+
+// A-count of the merge-path split of diagonal d in the window A = key[la, la+wa), B = key[lb, lb+wb)
+// (ties go to A, as in the merge step).
+__device__ __forceinline__ uint32_t mp_split(const uint32_t *key, uint32_t la, uint32_t wa, uint32_t lb, uint32_t wb,
+                                             uint32_t d)
+{
+    uint32_t lo = (d > wb) ? d - wb : 0u, hi = (d < wa) ? d : wa;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key[la + mid] <= key[lb + d - 1 - mid]) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Bytes of x that are zero -> 0x80 in that byte, else 0 (exact, no carries across bytes).
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x)
+{
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+
+// Block intersection of two short sorted windows A[0, na), B[0, nb) in LDS: W = 4 blocks, all 16
+// pairs compared, then the block with the smaller (or equal) maximum advances by 4 -- the block
+// rule of shuffling.rs:42-66, with partial blocks masked instead of a scalar tail.  QFILTER first
+// compares the blocks' low bytes (four rotations of one packed word, SWAR) and skips the full
+// compares when no byte pair agrees -- qfilter's byte-check prefilter (qfilter.rs:32-89) in 32-bit
+// SWAR form.  Returns bit i set when A[i] matched (BSR: when the state AND is non-zero); nelems
+// accumulates the popcount of the emitted states (BSR).
+template <bool QF, bool BSR>
+__device__ __forceinline__ uint32_t block_intersect(const uint32_t *key, const uint32_t *st, uint32_t ia,
+                                                    uint32_t na, uint32_t ib, uint32_t nb, uint32_t &nelems)
+{
+    uint32_t mask = 0, i = 0, j = 0;
+    while (i < na && j < nb) {
+        uint32_t a[4], b[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            a[x] = key[ia + i + x];
+            b[x] = key[ib + j + x];
+        }
+        const uint32_t av = (na - i < 4u) ? na - i : 4u, bv = (nb - j < 4u) ? nb - j : 4u;
+        const uint32_t amax = key[ia + i + av - 1], bmax = key[ib + j + bv - 1];
+        bool any = true;
+        if constexpr (QF) {
+            const uint32_t alo = (a[0] & 0xFFu) | (a[1] & 0xFFu) << 8 | (a[2] & 0xFFu) << 16 | a[3] << 24;
+            const uint32_t blo = (b[0] & 0xFFu) | (b[1] & 0xFFu) << 8 | (b[2] & 0xFFu) << 16 | b[3] << 24;
+            uint32_t cand = 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) cand |= zero_bytes(alo ^ ((blo >> (8 * r)) | (blo << ((32 - 8 * r) & 31))));
+            any = cand != 0;
+        }
+        if (any) {
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                uint32_t hit = 0, stv = 0;
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const bool eq = (uint32_t)x < av && (uint32_t)y < bv && a[x] == b[y];
+                    if constexpr (BSR) stv |= eq ? (st[ia + i + x] & st[ib + j + y]) : 0u;
+                    else hit |= eq ? 1u : 0u;
+                }
+                if constexpr (BSR) {
+                    hit = stv != 0 ? 1u : 0u;
+                    nelems += __popc(stv);
+                }
+                mask |= hit << (i + x);
+            }
+        }
+        i += (amax <= bmax) ? 4u : 0u;
+        j += (bmax <= amax) ? 4u : 0u;
+    }
+    return mask;
+}
+
 // Count + stage: one workgroup per tile (tiles tile0 + blockIdx.x), no communication between
 // workgroups.  Each tile's results go to its own staging slot (or nowhere in count-only mode);
 // its counts go to tile_count / tile_elems; per-pair counts and in-tile offsets are recorded for
 // the placement kernel.
-template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER>
+template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP = LP_MERGE>
 __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
     static_assert(K >= 1 && K <= 31, "lane masks hold K bits");
     constexpr int U = NT * K;
-    using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER>;
+    constexpr bool BLOCKP = LP != LP_MERGE;
+    using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
     __shared__ Smem sm;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -162,9 +243,52 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         // the lane moved to the next pair at step kt, from that pair's window start (pa1).
         const uint32_t ub = tid * K;
         uint32_t smask = 0, tmask = 0, pa0 = 0, pa1 = 0;
-        uint32_t svals[BSR ? K : 1];
+        uint32_t svals[(BSR && !BLOCKP) ? K : 1];
         uint32_t nelems = 0;
-        uint32_t kmax = 0, kt = K;  // steps to run; step at which the lane enters the next pair
+        uint32_t kt = K;  // merge policy: step at which the lane enters the next pair
+        // block policies: smask bit x is the lane's x-th A element (first sub-window, then the
+        // next pair's); pa0 / pa1 locate them, pb0/nb0 and pb1/nb1 the B sub-windows (BSR output)
+        uint32_t nfirst = 0, pb0 = 0, nb0 = 0, pb1 = 0, nb1 = 0;
+        if constexpr (BLOCKP) {
+            if (ub < ulen) {
+                uint32_t lo = 0, hi = nseg;  // largest s with seg_u[s] <= ub
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (sm.seg_u[mid] <= ub) lo = mid; else hi = mid;
+                }
+                const uint32_t ue = (ulen - ub < (uint32_t)K) ? ulen : ub + K;
+                for (uint32_t part = 0, s = lo; part < 2; part++, s++) {
+                    const uint32_t su = sm.seg_u[s];
+                    const uint32_t sn = sm.seg_u[s + 1];
+                    if (part == 1 && ue <= su) break;
+                    const uint32_t la = sm.seg_la[s], lb = sm.seg_lb[s];
+                    const uint32_t wa = sm.seg_wa[s], wb = sm.seg_wb[s], wn = wa + wb;
+                    uint32_t d0 = (ub > su) ? ub - su : 0u, d1 = ((ue < sn) ? ue : sn) - su;
+                    d0 = d0 < wn ? d0 : wn;
+                    d1 = d1 < wn ? d1 : wn;
+                    const uint32_t i0 = mp_split(sm.key, la, wa, lb, wb, d0);
+                    const uint32_t i1 = mp_split(sm.key, la, wa, lb, wb, d1);
+                    const uint32_t j0 = d0 - i0, j1 = d1 - i1;
+                    const uint32_t nbw = j1 - j0 + ((j1 < sm.seg_nb[s]) ? 1u : 0u);  // + boundary element
+                    const uint32_t m = block_intersect<LP == LP_QFILTER, BSR>(sm.key, sm.st, la + i0, i1 - i0,
+                                                                              lb + j0, nbw, nelems);
+                    if (part == 0) {
+                        smask = m;
+                        nfirst = i1 - i0;
+                        pa0 = la + i0;
+                        pb0 = lb + j0;
+                        nb0 = nbw;
+                    } else {
+                        smask |= m << nfirst;
+                        pa1 = la + i0;
+                        pb1 = lb + j0;
+                        nb1 = nbw;
+                    }
+                    if (ue <= sn) break;
+                }
+            }
+        } else {
+        uint32_t kmax = 0;  // steps to run
         uint32_t pa = 0, pb = 0, ea = 0, eb = 0, ebp = 0, s = 0;
         if (ub < ulen) {
             uint32_t lo = 0, hi = nseg;  // largest s with seg_u[s] <= ub (seg_u[0] == 0)
@@ -242,6 +366,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                 if ((uint32_t)k < kmax) step((uint32_t)k);
             }
         }
+        }
         const uint32_t nwords = __popc(smask);
         if constexpr (!BSR) nelems = nwords;
 
@@ -250,6 +375,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         const uint32_t pfx = block_exclusive_scan<NT>(nwords, sm.scan_scratch, &tot);
         sm.lane_mask[tid] = smask;
         sm.lane_pfx[tid] = pfx;
+        if constexpr (BLOCKP) sm.lane_nfirst[tid] = nfirst;
         if constexpr (BSR) {
             uint32_t el = nelems;
 #pragma unroll
@@ -268,7 +394,24 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                 return (k < kt) ? pa0 + __popc(before) : pa1 + __popc(before & ~kt_mask);
             };
             uint32_t pos = pfx;
-            if constexpr (BSR) {
+            if constexpr (BLOCKP) {
+                uint32_t sm_ = smask;
+                while (sm_) {
+                    const uint32_t x = __builtin_ctz(sm_);
+                    sm_ &= sm_ - 1;
+                    const uint32_t ai = (x < nfirst) ? pa0 + x : pa1 + (x - nfirst);
+                    const uint32_t v = sm.key[ai];
+                    dk[pos] = v;
+                    if constexpr (BSR) {
+                        // the matching B base lies in the same sub-window: find it again
+                        const uint32_t jb = (x < nfirst) ? pb0 : pb1, nbw = (x < nfirst) ? nb0 : nb1;
+                        uint32_t stv = 0;
+                        for (uint32_t y = 0; y < nbw; y++) stv |= (sm.key[jb + y] == v) ? sm.st[jb + y] : 0u;
+                        ds[pos] = sm.st[ai] & stv;
+                    }
+                    pos++;
+                }
+            } else if constexpr (BSR) {
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     if (smask & (1u << k)) {
@@ -298,10 +441,14 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             plan.tile_elems[t] = el;
         }
         if (tid < nseg) {
+            // results of the tile's units before x: whole lanes, then the lane's results before x
+            // (merge: steps before x; block policies: the lane's A elements before a pair boundary)
             auto before = [&](uint32_t x) -> uint32_t {
                 if (x >= ulen) return tot;
                 const uint32_t l = x / K, k = x % K;
-                return sm.lane_pfx[l] + __popc(sm.lane_mask[l] & ((1u << k) - 1u));
+                uint32_t nb_ = k;
+                if constexpr (BLOCKP) nb_ = (k == 0) ? 0u : sm.lane_nfirst[l];
+                return sm.lane_pfx[l] + __popc(sm.lane_mask[l] & ((1u << nb_) - 1u));
             };
             const uint32_t c0v = before(sm.seg_u[tid]);
             const uint32_t c = before(sm.seg_u[tid + 1]) - c0v;

@@ -65,7 +65,7 @@ static_assert(kC0 >= (uint32_t)kK && kC0S >= (uint32_t)kKS, "a lane crosses at m
 
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
 
-enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2 };
+enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4 };
 
 struct AlgoEntry {
     const char *name;
@@ -82,6 +82,10 @@ const AlgoEntry kAlgos[] = {
     {"hip_galloping", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_GALLOP, false},
     {"hip_galloping_bsr", SETOPS_ALGO_BSR, KIND_GALLOP, true},
     {"hip_binary_search", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_BINARY, false},
+    {"hip_shuffling", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_SHUFFLE, false},
+    {"hip_shuffling_bsr", SETOPS_ALGO_BSR, KIND_SHUFFLE, true},
+    {"hip_qfilter", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_QFILTER, false},
+    {"hip_qfilter_bsr", SETOPS_ALGO_BSR, KIND_QFILTER, true},
 };
 constexpr int kNumAlgos = (int)(sizeof(kAlgos) / sizeof(kAlgos[0]));
 
@@ -173,18 +177,20 @@ namespace {
 
 int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles);
 
-uint32_t policy_of(Kind k) { return k == KIND_MERGE ? POLICY_MERGE : POLICY_SEARCH; }
-uint32_t units_per_tile(Kind k) { return k == KIND_MERGE ? (uint32_t)kU : (uint32_t)kUS; }
-uint32_t pad_units(Kind k) { return k == KIND_MERGE ? kC0 : kC0S; }
+// Merge-path kinds (merge, shuffling, qfilter) share the merge tile; the search kinds the search tile.
+bool merge_like(Kind k) { return k == KIND_MERGE || k == KIND_SHUFFLE || k == KIND_QFILTER; }
+uint32_t policy_of(Kind k) { return merge_like(k) ? POLICY_MERGE : POLICY_SEARCH; }
+uint32_t units_per_tile(Kind k) { return merge_like(k) ? (uint32_t)kU : (uint32_t)kUS; }
+uint32_t pad_units(Kind k) { return merge_like(k) ? kC0 : kC0S; }
 // Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
 // can produce (merge: (U+1)/2 + 1; search: U).
 uint32_t stage_words(Kind k, bool bsr)
 {
-    const uint32_t half = (k == KIND_MERGE ? (uint32_t)kU / 2 : (uint32_t)kUS) + 64;
+    const uint32_t half = (merge_like(k) ? (uint32_t)kU / 2 : (uint32_t)kUS) + 64;
     return bsr ? 2 * half : half;
 }
 // The plan's unit prefix has a closed form only for merge-policy CSR batches (and single pairs).
-bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gather && h->kind == KIND_MERGE); }
+bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gather && merge_like(h->kind)); }
 
 // Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields.
 int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
@@ -333,11 +339,11 @@ int device_unit_total(setops_hip_batch *h, uint64_t *utotal)
 
 // Tile kernels run one workgroup per tile; batches of more than kMaxGrid tiles take several
 // launches (tile offset tile0).
-template <bool BSR, bool CO, bool GA>
+template <bool BSR, bool CO, bool GA, int LP>
 void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO, GA>),
+        hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO, GA, LP>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
 }
@@ -356,8 +362,16 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     switch (h->kind) {
     case KIND_MERGE:
-        if (h->gather) launch_merge_t<BSR, CO, true>(h, out, stream);
-        else launch_merge_t<BSR, CO, false>(h, out, stream);
+        if (h->gather) launch_merge_t<BSR, CO, true, LP_MERGE>(h, out, stream);
+        else launch_merge_t<BSR, CO, false, LP_MERGE>(h, out, stream);
+        break;
+    case KIND_SHUFFLE:
+        if (h->gather) launch_merge_t<BSR, CO, true, LP_SHUFFLE>(h, out, stream);
+        else launch_merge_t<BSR, CO, false, LP_SHUFFLE>(h, out, stream);
+        break;
+    case KIND_QFILTER:
+        if (h->gather) launch_merge_t<BSR, CO, true, LP_QFILTER>(h, out, stream);
+        else launch_merge_t<BSR, CO, false, LP_QFILTER>(h, out, stream);
         break;
     case KIND_GALLOP: launch_search_t<BSR, CO, true>(h, out, stream); break;
     case KIND_BINARY: launch_search_t<BSR, CO, false>(h, out, stream); break;
@@ -480,7 +494,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->in.contig = gather ? 0u : 1u;
     h->plan.single = 0;
     int rc;
-    if (!gather && e->kind == KIND_MERGE) {
+    if (!gather && merge_like(e->kind)) {
         rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)kC0 * npairs);
     } else {
         // size the scan arrays first, then count units on the device
@@ -552,7 +566,8 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     }
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
-    const uint64_t units = (e->kind == KIND_MERGE ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind);
+    const uint64_t units =
+        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind);
     h->plan.single = 1;
     int rc = batch_reserve(h, 1, units);
     if (rc != SETOPS_OK) return rc;

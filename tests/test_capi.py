@@ -52,6 +52,29 @@ def test_registry_without_device():
     assert lib.setops_hip_algo_flags(-1) == 0
 
 
+# Every 2-set entry of the reference's registry that this path covers has a GPU twin
+# (benchmark/src/timer.rs:80-167 plain names, :212-254 BSR names).
+REFERENCE_TWINS = {
+    "branchless_merge": _native.ALGO_TWOSET | _native.ALGO_KSET,
+    "galloping": _native.ALGO_TWOSET | _native.ALGO_KSET,
+    "binary_search": _native.ALGO_TWOSET | _native.ALGO_KSET,
+    "shuffling": _native.ALGO_TWOSET | _native.ALGO_KSET,
+    "qfilter": _native.ALGO_TWOSET | _native.ALGO_KSET,
+    "branchless_merge_bsr": _native.ALGO_BSR,
+    "galloping_bsr": _native.ALGO_BSR,
+    "shuffling_bsr": _native.ALGO_BSR,
+    "qfilter_bsr": _native.ALGO_BSR,
+}
+
+
+def test_registry_covers_reference_entries():
+    lib = _native.hip()
+    for ref, flags in REFERENCE_TWINS.items():
+        i = lib.setops_hip_algo_lookup(("hip_" + ref).encode())
+        assert i >= 0, ref
+        assert lib.setops_hip_algo_flags(i) == flags, ref
+
+
 def test_unknown_algorithm_is_an_error():
     lib = _native.hip()
     assert lib.setops_hip_algo_lookup(b"branchless_merge_but_wrong") == _native.SETOPS_ERR_UNKNOWN_ALGO

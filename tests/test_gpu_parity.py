@@ -14,8 +14,8 @@ from simd_set_operations_amd.visitor import Counter, VecWriter
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
-PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search"]
-BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr"]
+PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search", "hip_shuffling", "hip_qfilter"]
+BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr"]
 
 
 def announce(msg):
@@ -260,7 +260,7 @@ def test_batch_bsr(name):
     db.close()
 
 
-@pytest.mark.parametrize("name", ["hip_branchless_merge", "hip_galloping"])
+@pytest.mark.parametrize("name", ["hip_branchless_merge", "hip_galloping", "hip_qfilter"])
 def test_uneven_batch_full_size(name):
     """The 65,536-pair uneven workload: per-pair counts equal the planted shared counts (a
     size-independent property of the generator), every output run is strictly increasing, and
