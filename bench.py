@@ -25,11 +25,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+MERGE_KERNEL = "setops::merge_tile_kernel<256, 16, 130, false, false, false, 0>"
 
 
 def log(msg):
     sys.stderr.write(f"[bench] {msg}\n")
     sys.stderr.flush()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "merge_tile_kernel_uneven_batch_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size); None when absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(float(line.split(":")[-1]))
+    except OSError:
+        pass
+    return None
 
 
 def algorithmic_bytes(E, out_total, P):
@@ -330,8 +347,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "merge_tile_kernel",
+            "traffic": measured_traffic(MERGE_KERNEL),
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+            "kernel": MERGE_KERNEL,
             "kernel_ms": kern_ms,
             "algorithmic_bytes": Bytes,
         },
