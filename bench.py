@@ -197,6 +197,97 @@ def skewed_leg(torch, steps, cpu_threads, with_cpu):
     return res
 
 
+def graph_leg(torch, cpu_threads, with_cpu, scale):
+    """The graph workload: R-MAT (0.57, 0.19, 0.19, 0.05) at 2^scale vertices, edge factor 16,
+    symmetrized and deduplicated.  For every edge u < v, |N(u) & N(v)| summed (= 3T), count-only,
+    on the CSR in place (gather batches): plain merge, BSR merge (popcount sum), the oriented lists
+    (= T, the independent check 3 x T), and a skew split -- galloping for edges whose larger list
+    exceeds `threshold` x the smaller, merge for the rest -- over a small threshold sweep (the
+    threshold is autotuned here, on this graph).  Throughput uses the streamed element count
+    sum_{u<v} (d_u + d_v) = sum_v d_v^2."""
+    from simd_set_operations_amd import datagen
+    from simd_set_operations_amd.graph import EdgeIntersections, csr_to_bsr, split_by_skew
+
+    t0 = time.perf_counter()
+    rp, col = datagen.rmat_graph(scale=scale)
+    u, v = datagen.graph_edges(rp, col)
+    up = datagen.upper_starts(rp, col)
+    brp, bases, states = csr_to_bsr(rp, col)
+    deg = np.diff(rp).astype(np.int64)
+    E = int((deg[u] + deg[v]).sum())
+    res = {"scale": scale, "vertices": int(rp.size - 1), "edges_u_lt_v": int(u.size),
+           "avg_degree": float(col.size / (rp.size - 1)), "max_degree": int(deg.max()), "elements_streamed": E,
+           "bsr_words": int(bases.size), "gen_seconds": time.perf_counter() - t0}
+    log(f"graph: {res}")
+    dcol = torch.from_numpy(col.view(np.int32)).cuda()
+    dbases = torch.from_numpy(bases.view(np.int32)).cuda()
+    dstates = torch.from_numpy(states.view(np.int32)).cuda()
+    stream = torch.cuda.current_stream()
+
+    def timed(batches, reps=2):
+        for b in batches:
+            b.run(stream)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            for b in batches:
+                b.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        tot = [b.totals() for b in batches]
+        return ev0.elapsed_time(ev1) / reps, sum(t[0] for t in tot), sum(t[1] for t in tot)
+
+    e = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v)
+    ms, s3, _ = timed([e])
+    e.close()
+    res["merge"] = {"ms": ms, "sum": s3, "gelems": E / ms / 1e6}
+    e = EdgeIntersections("hip_branchless_merge", dcol, up, rp[1:], u, v)
+    ms, t, _ = timed([e])
+    e.close()
+    res["oriented_merge"] = {"ms": ms, "triangles": t}
+    e = EdgeIntersections("hip_branchless_merge_bsr", dbases, brp[:-1], brp[1:], u, v, states=dstates)
+    ms, _, s3b = timed([e])
+    e.close()
+    res["merge_bsr"] = {"ms": ms, "sum": s3b, "gelems": E / ms / 1e6}
+    best = None
+    for thr in (32.0, 128.0, 512.0):
+        sk = split_by_skew(deg[u], deg[v], thr)
+        a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk)
+        b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
+        ms, sh, _ = timed([a, b])
+        a.close()
+        b.close()
+        res[f"split_{int(thr)}"] = {"ms": ms, "sum": sh, "search_edges": int(sk.sum()), "gelems": E / ms / 1e6}
+        if sh != s3:
+            log(f"graph split {thr}: sum {sh} differs from {s3}")
+            sys.exit(3)
+        if best is None or ms < res[best]["ms"]:
+            best = f"split_{int(thr)}"
+    res["best_split"] = best
+    if s3 != 3 * t or s3b != s3:
+        log(f"graph: sums disagree (merge {s3}, bsr {s3b}, oriented {t})")
+        sys.exit(3)
+    res["triangles"] = t
+    log(f"graph results: { {k: res[k] for k in res if k.startswith(('merge', 'oriented', 'split'))} }")
+    if with_cpu:
+        import oracle
+
+        rng = np.random.Generator(np.random.PCG64(7))
+        sel = np.sort(rng.choice(u.size, size=max(1, u.size // 200), replace=False))
+        su, sv = u[sel], v[sel]
+        Es = int((deg[su] + deg[sv]).sum())
+        t0 = time.perf_counter()
+        oracle.batch_intersect_gather("branchless_merge", col, rp[su], rp[su + 1], col, rp[sv], rp[sv + 1],
+                                      nthreads=cpu_threads, with_output=False)
+        el = time.perf_counter() - t0
+        res["cpu_port_branchless_merge"] = {"sample_edges": int(sel.size), "sample_elements": Es, "seconds": el,
+                                            "threads": cpu_threads, "gelems": Es / el / 1e9}
+        log(f"graph cpu: {res['cpu_port_branchless_merge']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +298,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=2048)
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-skewed", action="store_true", help="skip the skewed resident-set leg")
+    ap.add_argument("--no-graph", action="store_true", help="skip the R-MAT graph leg")
+    ap.add_argument("--graph-scale", type=int, default=22)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the reference kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -315,6 +408,9 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_graph:
+        extra["graph"] = graph_leg(torch, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines,
+                                   args.graph_scale)
     if not args.no_skewed:
         extra["skewed"] = skewed_leg(torch, args.steps, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines)
 

@@ -201,4 +201,66 @@ int setops_gen_skewed(uint64_t seed, uint64_t universe, uint64_t large_n, uint64
     return bad ? -1 : 0;
 }
 
+// R-MAT graph (Chakrabarti, Zhan & Faloutsos, SDM 2004; Graph500 parameters): edge_factor << scale
+// edges, each placing itself by `scale` quadrant choices with probabilities (a, b, c, 1-a-b-c);
+// self loops dropped, both directions kept, duplicates removed, neighbour lists sorted.
+// rowptr has (1 << scale) + 1 entries; col needs room for 2 * (edge_factor << scale) entries.
+// Returns the number of stored (directed) entries, or -1.
+int64_t setops_gen_rmat(uint64_t seed, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                        uint64_t *rowptr, uint32_t *col)
+{
+    if (scale == 0 || scale > 30 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return -1;
+    const uint64_t n = 1ull << scale, m = (uint64_t)edge_factor << scale;
+    std::vector<uint32_t> src(m), dst(m);
+    const uint64_t chunk = 1 << 16;
+#pragma omp parallel for schedule(static)
+    for (int64_t ch = 0; ch < (int64_t)((m + chunk - 1) / chunk); ch++) {
+        SplitMix64 rng(stream_seed(seed, (uint64_t)ch));
+        const uint64_t e0 = (uint64_t)ch * chunk, e1 = std::min<uint64_t>(m, e0 + chunk);
+        for (uint64_t e = e0; e < e1; e++) {
+            uint32_t u = 0, v = 0;
+            for (uint32_t l = 0; l < scale; l++) {
+                const double r = rng.unit();
+                const uint32_t qu = r >= a + b ? 1u : 0u;                 // lower half (c or d)
+                const uint32_t qv = (r >= a && r < a + b) || r >= a + b + c ? 1u : 0u;  // right half (b or d)
+                u = (u << 1) | qu;
+                v = (v << 1) | qv;
+            }
+            src[e] = u;
+            dst[e] = v;
+        }
+    }
+    // degrees of the symmetrized multigraph (self loops dropped)
+    std::vector<uint64_t> deg(n + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)m; e++) {
+        if (src[e] == dst[e]) continue;
+        __atomic_fetch_add(&deg[src[e]], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&deg[dst[e]], 1, __ATOMIC_RELAXED);
+    }
+    std::vector<uint64_t> start(n + 1, 0);
+    for (uint64_t v = 0; v < n; v++) start[v + 1] = start[v] + deg[v];
+    std::vector<uint64_t> cur(start.begin(), start.end() - 1);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)m; e++) {
+        const uint32_t u = src[e], v = dst[e];
+        if (u == v) continue;
+        col[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
+        col[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+    }
+    // sort + dedup each list in place, then compact
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t v = 0; v < (int64_t)n; v++) {
+        uint32_t *l = col + start[v];
+        const uint64_t d = start[v + 1] - start[v];
+        std::sort(l, l + d);
+        deg[v] = (uint64_t)(std::unique(l, l + d) - l);
+    }
+    rowptr[0] = 0;
+    for (uint64_t v = 0; v < n; v++) rowptr[v + 1] = rowptr[v] + deg[v];
+    for (uint64_t v = 0; v < n; v++)  // sequential: destinations never pass their sources
+        if (rowptr[v] != start[v]) std::memmove(col + rowptr[v], col + start[v], deg[v] * sizeof(uint32_t));
+    return (int64_t)rowptr[n];
+}
+
 } // extern "C"

@@ -8,6 +8,8 @@ The workloads are the benchmark configurations of this path:
   with r log-uniform in [1, 1024]; element space = 10 x larger; shared = round(sel x smaller),
   sel ~ U[0,1].
 
+* ``rmat_graph``   -- R-MAT (a, b, c, d) = (0.57, 0.19, 0.19, 0.05), 2^22 vertices, 16 x 2^22
+  sampled edges, symmetrized and deduplicated, as a CSR of sorted u32 neighbour lists.
 * ``skewed``       -- one resident large set of 2^24 over {0..2^28-1} against 4,096 small sets of
   2^8, each with round(0.1 x 256) = 26 elements drawn from the large set and the rest from its
   complement.
@@ -25,6 +27,7 @@ _c = ctypes
 
 PLAIN_PAIR = dict(universe=1 << 20, size=1 << 16, shared=19661, seed=1001)
 SKEWED = dict(universe=1 << 28, large=1 << 24, nsmall=4096, small=256, shared=26, seed=1003)
+RMAT = dict(scale=22, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1004)
 UNEVEN = dict(npairs=65536, lmin=6.0, lmax=14.0, log2_skew=10.0, density=0.1, seed=1002)
 
 
@@ -87,3 +90,35 @@ def skewed(seed=SKEWED["seed"], universe=SKEWED["universe"], large=SKEWED["large
     if rc != 0:
         raise ValueError("skewed workload does not fit its element space")
     return L, S
+
+
+def rmat_graph(scale=RMAT["scale"], edge_factor=RMAT["edge_factor"], a=RMAT["a"], b=RMAT["b"], c=RMAT["c"],
+               seed=RMAT["seed"]):
+    """R-MAT graph as CSR: (rowptr[n+1] u64, col[nnz] u32), lists sorted and duplicate-free,
+    no self loops, symmetric."""
+    n = 1 << scale
+    m = edge_factor << scale
+    rowptr = np.zeros(n + 1, np.uint64)
+    col = np.zeros(2 * m, np.uint32)
+    nnz = _native.datagen().setops_gen_rmat(seed, scale, edge_factor, a, b, c, _p(rowptr), _p(col))
+    if nnz < 0:
+        raise ValueError("bad R-MAT parameters")
+    return rowptr, col[:nnz].copy()
+
+
+def graph_edges(rowptr, col):
+    """Edges (u, v) with u < v of a symmetric CSR, in CSR order."""
+    n = rowptr.size - 1
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rowptr).astype(np.int64))
+    keep = src < col
+    return src[keep], col[keep]
+
+
+def upper_starts(rowptr, col):
+    """For every vertex u, the index in col of its first neighbour greater than u (the start of
+    the oriented list N+(u))."""
+    n = rowptr.size - 1
+    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(rowptr).astype(np.int64))
+    keys = (src << np.uint64(32)) | col.astype(np.uint64)
+    probe = (np.arange(n, dtype=np.uint64) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    return np.searchsorted(keys, probe, side="right").astype(np.uint64)

@@ -376,3 +376,53 @@ def test_skewed_shared_large_set(name):
     for p in range(P):
         assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], np.intersect1d(smalls[p], large)), p
     gb.close()
+
+
+# ---- graph edges (count-only gather batches) ------------------------------------------------------
+def test_graph_triangle_sums():
+    """R-MAT edges: per-edge counts of every entry equal the oracle's; full-list sums are three times
+    the oriented sum; the BSR form gives the same sum through popcounts; a skew split (search for
+    hub-leaf edges, merge for the rest) adds up to the same total."""
+    import torch
+
+    from simd_set_operations_amd import datagen
+    from simd_set_operations_amd.batch import GatherBatch
+    from simd_set_operations_amd.graph import EdgeIntersections, csr_to_bsr, split_by_skew
+
+    rp, col = datagen.rmat_graph(scale=13)
+    u, v = datagen.graph_edges(rp, col)
+    up = datagen.upper_starts(rp, col)
+    brp, bases, states = csr_to_bsr(rp, col)
+    exp, _, _ = oracle.batch_intersect_gather("branchless_merge", col, rp[u], rp[u + 1], col, rp[v], rp[v + 1],
+                                              nthreads=8, with_output=False)
+    s3 = int(exp.sum())
+    announce(f"graph scale=13 edges={u.size} 3T={s3}")
+    dcol = torch.from_numpy(col.view(np.int32)).cuda()
+    dbases = torch.from_numpy(bases.view(np.int32)).cuda()
+    dstates = torch.from_numpy(states.view(np.int32)).cuda()
+    for name in PLAIN:
+        gb = GatherBatch(name, dcol, rp[u], rp[u + 1], dcol, rp[v], rp[v + 1], outputs=False)
+        gb.run(count_only=True)
+        torch.cuda.synchronize()
+        assert gb.totals()[0] == s3, name
+        assert np.array_equal(gb.d_counts.cpu().numpy().view(np.uint32), exp), name
+        gb.close()
+    ori = EdgeIntersections("hip_branchless_merge", dcol, up, rp[1:], u, v)
+    ori.run()
+    assert 3 * ori.totals()[0] == s3
+    ori.close()
+    for name in BSR:
+        e = EdgeIntersections(name, dbases, brp[:-1], brp[1:], u, v, states=dstates)
+        e.run()
+        assert e.totals()[1] == s3, name
+        e.close()
+    deg = np.diff(rp).astype(np.int64)
+    sk = split_by_skew(deg[u], deg[v], 16.0)
+    assert 0 < sk.sum() < sk.size
+    a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk)
+    b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
+    a.run()
+    b.run()
+    assert a.totals()[0] + b.totals()[0] == s3
+    a.close()
+    b.close()
