@@ -288,6 +288,47 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     return res
 
 
+def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
+    """The served k-set workload: a device-resident corpus of 2^20 sorted lists (bounded-Zipf(1.1)
+    lengths in [2^4, 2^18] over [0, 2^26)); queries of k ~ U{2..6} lists by Zipf(1.0) popularity,
+    answered by SvS smallest-first, arriving as Poisson streams.  The continuous-batching engine
+    (every in-flight query's next round in one gather batch per step, admission by arena budget) is
+    compared with the plain path (one query at a time, one launch sequence per round) on the same
+    stream.  Per-query elements follow the reference's convention (sum of the k list sizes)."""
+    from simd_set_operations_amd import serving
+
+    t0 = time.perf_counter()
+    corpus = serving.Corpus(mode=mode)
+    res = {"lists": corpus.nlists, "length_mode": mode, "elements": int(corpus.lens.sum()),
+           "corpus_gb": corpus.total * 4 / 1e9, "gen_seconds": time.perf_counter() - t0}
+    log(f"serving corpus: {res}")
+    eng = serving.Engine(corpus, "hip_galloping")
+    ids, ks = serving.make_queries(corpus, 4000, seed=11)
+    eng.serve(ids, ks, np.zeros(ids.shape[0]))  # warm up
+    for i, rate in enumerate(rates):
+        nq = int(rate * seconds_per_rate)
+        ids, ks = serving.make_queries(corpus, nq, seed=100 + i)
+        arr = serving.poisson_arrivals(nq, rate, seed=200 + i)
+        lens = np.where(ids >= 0, corpus.lens[np.maximum(ids, 0)], 0)
+        E = int(lens.sum())
+        wids, wks = serving.make_queries(corpus, 2000, seed=300 + i)  # warm the engine at this rate
+        eng.serve(wids, wks, serving.poisson_arrivals(2000, rate, seed=400 + i))
+        lat, _, st = eng.serve(ids, ks, arr)
+        r = {"queries": nq, "engine": serving.latency_summary(lat), "engine_steps": st["steps"],
+             "engine_pairs": st["pairs"], "engine_qps": st["served"] / st["wall_s"],
+             "engine_gelems": E / st["wall_s"] / 1e9}
+        plat, _, pst = serving.plain_serve(corpus, ids, ks, arr, timeout_s=plain_budget_s)
+        r["plain"] = serving.latency_summary(plat)
+        r["plain_qps"] = pst["served"] / pst["wall_s"]
+        r["plain_saturated"] = pst["served"] < nq
+        res[f"{int(rate)}qps"] = r
+        log(f"serving {int(rate)} qps: {r}")
+    eng.close()
+    del corpus
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +341,9 @@ def main():
     ap.add_argument("--no-skewed", action="store_true", help="skip the skewed resident-set leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the R-MAT graph leg")
     ap.add_argument("--graph-scale", type=int, default=22)
+    ap.add_argument("--no-serving", action="store_true", help="skip the served k-set leg")
+    ap.add_argument("--serving-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
+    ap.add_argument("--corpus-mode", choices=("iid", "rank"), default="iid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the reference kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -408,6 +452,8 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_serving:
+        extra["serving"] = serving_leg(torch, (5e4, 2e5, 8e5), args.serving_seconds, 5.0, args.corpus_mode)
     if not args.no_graph:
         extra["graph"] = graph_leg(torch, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines,
                                    args.graph_scale)

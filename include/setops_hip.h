@@ -129,6 +129,16 @@ int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const ui
                                    const uint64_t *d_a_end, const uint32_t *d_b, const uint32_t *d_b_states,
                                    uint64_t b_len, const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs);
 
+/* Points a gather batch at new pairs (same pools and algorithm), reusing its device workspace
+ * (grow-only: no allocation once it is large enough).  h_na / h_nb are the pairs' sizes on the host
+ * -- they must equal d_a_end - d_a_beg and d_b_end - d_b_beg -- so nothing is read back from the
+ * device.  For request loops that batch different pairs every step (continuous batching over a
+ * resident corpus); the reference has no counterpart.  The previous run must be complete or on the
+ * same stream as the next one. */
+int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, const uint64_t *d_a_end,
+                            const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs, const uint64_t *h_na,
+                            const uint64_t *h_nb);
+
 /* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
  *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
  *                          both NULL selects count-only mode (out_cap ignored)

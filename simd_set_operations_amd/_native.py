@@ -10,6 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(_HERE, "libsetops_hip.so")
 DATAGEN_LIB_PATH = os.path.join(_HERE, "libsetops_datagen.so")
+DATAGEN_HIP_LIB_PATH = os.path.join(_HERE, "libsetops_datagen_hip.so")
 
 SETOPS_OK = 0
 SETOPS_ERR_INVALID = -1
@@ -75,6 +76,7 @@ HIP_SIGNATURES = {
         _c.c_int,
         [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp, _c.c_uint64],
     ),
+    "setops_hip_batch_rebind": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp]),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),
     "setops_hip_batch_set_timing": (_c.c_int, [_vp, _c.c_int]),
@@ -107,8 +109,13 @@ DATAGEN_SIGNATURES = {
     ),
 }
 
+DATAGEN_HIP_SIGNATURES = {
+    "setops_gen_sorted_lists_dev": (_c.c_int, [_c.c_uint64, _c.c_uint64, _vp, _vp, _c.c_uint64, _vp, _vp]),
+}
+
 _hip = None
 _datagen = None
+_datagen_hip = None
 
 
 def _bind(path, sigs):
@@ -154,3 +161,15 @@ def check(rc):
         msg = hip().setops_hip_last_error()
         raise SetopsError(rc, msg.decode() if msg else "")
     return rc
+
+
+def datagen_hip():
+    """Device-side workload generator (imports torch first, like hip())."""
+    global _datagen_hip
+    if _datagen_hip is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        _datagen_hip = _bind(DATAGEN_HIP_LIB_PATH, DATAGEN_HIP_SIGNATURES)
+    return _datagen_hip

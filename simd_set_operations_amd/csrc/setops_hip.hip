@@ -791,6 +791,33 @@ int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const ui
                         d_b_end, npairs);
 }
 
+int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, const uint64_t *d_a_end,
+                            const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs, const uint64_t *h_na,
+                            const uint64_t *h_nb)
+{
+    if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
+    if (!handle->gather) return fail(SETOPS_ERR_UNSUPPORTED, "only gather batches can be rebound");
+    if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
+    if (!d_a_beg || !d_a_end || !d_b_beg || !d_b_end || !h_na || !h_nb)
+        return fail(SETOPS_ERR_INVALID, "NULL argument");
+    const uint32_t c0 = pad_units(handle->kind);
+    uint64_t utotal = 0;
+    for (uint64_t p = 0; p < npairs; p++) {
+        if (h_na[p] >= 0xFFFFFFFFull || h_nb[p] >= 0xFFFFFFFFull)
+            return fail(SETOPS_ERR_INVALID, "pair %llu: sets must hold < 2^32 elements", (unsigned long long)p);
+        utotal += (merge_like(handle->kind) ? h_na[p] + h_nb[p] : std::min(h_na[p], h_nb[p])) + c0;
+    }
+    int rc = batch_reserve(handle, npairs, utotal);
+    if (rc != SETOPS_OK) return rc;
+    handle->P = npairs;
+    handle->in.P = npairs;
+    handle->in.a_beg = d_a_beg;
+    handle->in.a_end = d_a_end;
+    handle->in.b_beg = d_b_beg;
+    handle->in.b_end = d_b_end;
+    return SETOPS_OK;
+}
+
 int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_out_states, uint64_t out_cap,
                          uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals, void *stream)
 {

@@ -93,3 +93,29 @@ def test_graph_triangle_identity_on_the_oracle():
     A = np.zeros((n, n), np.int64)
     A[np.repeat(np.arange(n), np.diff(rp).astype(np.int64)), col] = 1
     assert np.trace(A @ A @ A) // 6 == t
+
+
+def test_served_workload_generators():
+    from types import SimpleNamespace
+
+    from simd_set_operations_amd import serving
+
+    lens = serving.zipf_lengths(1 << 16, 16, 1 << 18, 1.1, seed=5)
+    assert lens.min() >= 16 and lens.max() <= 1 << 18
+    # bounded Zipf(1.1) on [16, 2^18]: the mean follows from the pmf
+    sup = np.arange(16, (1 << 18) + 1, dtype=np.float64)
+    w = sup ** -1.1
+    mean = (sup * w).sum() / w.sum()
+    assert abs(lens.mean() / mean - 1) < 0.1
+    assert np.mean(lens == 16) > np.mean(lens == 17) > np.mean(lens == 64)
+    rk = serving.zipf_lengths(1000, 16, 1 << 18, 1.1, seed=5, mode="rank")
+    assert rk.max() == 1 << 18 and rk.min() == max(16, int(np.floor((1 << 18) * 1000.0 ** -1.1)))
+    c = SimpleNamespace(nlists=1000, lens=lens[:1000])
+    ids, ks = serving.make_queries(c, 500, seed=9)
+    assert ks.min() >= 2 and ks.max() <= 6 and set(np.unique(ks)) == {2, 3, 4, 5, 6}
+    for q in range(500):
+        row = ids[q, :ks[q]]
+        assert np.all(ids[q, ks[q]:] == -1) and np.unique(row).size == row.size
+        assert np.all(np.diff(c.lens[row].astype(np.int64)) >= 0)  # smallest first
+    t = serving.poisson_arrivals(20000, 1000.0, seed=3)
+    assert np.all(np.diff(t) > 0) and abs(t[-1] / 20.0 - 1) < 0.05

@@ -426,3 +426,41 @@ def test_graph_triangle_sums():
     assert a.totals()[0] + b.totals()[0] == s3
     a.close()
     b.close()
+
+
+# ---- served k-set queries -------------------------------------------------------------------------
+def test_served_queries_alone_and_in_the_mix():
+    """Continuous batching: every query's result equals the oracle's SvS over its lists, the plain
+    per-query path's result, and its result when served alone."""
+    from simd_set_operations_amd import serving
+
+    corpus = serving.Corpus(nlists=4096, lmin=16, lmax=1 << 12, universe=1 << 18, seed=77, arena_words=1 << 22)
+    for i in range(0, corpus.nlists, 511):
+        lst = corpus.list(i)
+        assert lst.size == corpus.lens[i]
+        assert np.all(np.diff(lst.astype(np.int64)) > 0) and (lst.size == 0 or lst[-1] < corpus.universe)
+    ids, ks = serving.make_queries(corpus, 1500, seed=78)
+    arrivals = serving.poisson_arrivals(ids.shape[0], 2e5, seed=79)
+    announce(f"serving nq={ids.shape[0]}")
+    eng = serving.Engine(corpus, "hip_galloping", max_pairs=256)
+    lat, res, stats = eng.serve(ids, ks, arrivals, keep_results=True)
+    assert stats["served"] == ids.shape[0]
+    assert stats["steps"] > 1 and stats["pairs"] >= ids.shape[0]
+    lists = {}
+
+    def L(i):
+        if i not in lists:
+            lists[i] = corpus.list(i)
+        return lists[i]
+
+    for q in range(ids.shape[0]):
+        sets = [L(int(i)) for i in ids[q, :ks[q]]]
+        exp = oracle.svs("branchless_merge", sets)
+        assert np.array_equal(res[q], exp), q
+    plat, pres, _ = serving.plain_serve(corpus, ids[:300], ks[:300], arrivals[:300] * 0, keep_results=True)
+    for q in range(300):
+        assert np.array_equal(pres[q], res[q]), q
+    for q in range(0, ids.shape[0], 100):  # alone: a stream of one query
+        _, r1, _ = eng.serve(ids[q:q + 1], ks[q:q + 1], np.zeros(1), keep_results=True)
+        assert np.array_equal(r1[0], res[q]), q
+    eng.close()
