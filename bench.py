@@ -25,7 +25,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-MERGE_KERNEL = "setops::merge_tile_kernel<256, 16, 130, false, false, false, 0>"
+# the kernel whose time is the roofline's (per headline entry)
+DOMINANT_KERNEL = {
+    "hip_branchless_merge": "setops::merge_tile_kernel<256, 16, 130, false, false, false, 0>",
+    "hip_shuffling": "setops::merge_tile_kernel<256, 16, 130, false, false, false, 1>",
+    "hip_qfilter": "setops::merge_tile_kernel<256, 16, 130, false, false, false, 2>",
+    "hip_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, true>",
+    "hip_binary_search": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, false>",
+}
 
 
 def log(msg):
@@ -335,6 +342,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--npairs", type=int, default=65536)
+    ap.add_argument("--algo", default="hip_branchless_merge", help="registered entry for the headline batch")
+    ap.add_argument("--compare", default="hip_qfilter,hip_shuffling,hip_galloping",
+                    help="comma-separated entries also timed on the headline batch (extras)")
     ap.add_argument("--cpu-budget", type=float, default=6.0, help="seconds of CPU baseline timing")
     ap.add_argument("--plain-sample", type=int, default=2048)
     ap.add_argument("--no-baselines", action="store_true")
@@ -380,59 +390,61 @@ def main():
     E = int(A.size + B.size)
     log(f"generated P={P} E={E} in {time.perf_counter() - t0:.1f}s")
 
-    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        db.run(stream)
-    torch.cuda.synchronize()
-    # correctness gate on the bench workload: per-pair counts equal the planted shared counts
-    off, cnt, _ = db.results()
-    if not np.array_equal(cnt, shared):
-        log("per-pair counts differ from the planted shared counts")
-        sys.exit(3)
-    out_total = int(off[-1])
-    log(f"warmup ok: {out_total} results, info={db.info()}")
 
-    db.kernel_time()  # reset
-    db.set_timing(True)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        db.run(stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    step_ms = ev0.elapsed_time(ev1) / args.steps
-    kern_ms_total, runs = db.kernel_time()
-    kern_ms = kern_ms_total / max(runs, 1)
-    db.set_timing(False)
+    def time_batch(algo):
+        """Warm up, check, then time `steps` runs with outputs and `steps` count-only runs."""
+        db = DeviceBatch(algo, A, a_off, B, b_off)
+        for _ in range(args.warmup):
+            db.run(stream)
+        torch.cuda.synchronize()
+        # correctness gate on the bench workload: per-pair counts equal the planted shared counts
+        off, cnt, _ = db.results()
+        if not np.array_equal(cnt, shared):
+            log(f"{algo}: per-pair counts differ from the planted shared counts")
+            sys.exit(3)
+        out_total = int(off[-1])
+        r = {"results": out_total, "info": db.info()}
+        for mode in ("out", "count"):
+            co = mode == "count"
+            for _ in range(2):
+                db.run(stream, count_only=co)
+            torch.cuda.synchronize()
+            db.kernel_time()  # reset
+            db.set_timing(True)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(args.steps):
+                db.run(stream, count_only=co)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            k_ms, runs = db.kernel_time()
+            db.set_timing(False)
+            r[mode] = {"ms_per_step": ev0.elapsed_time(ev1) / args.steps, "kernel_ms": k_ms / max(runs, 1)}
+            r[mode]["gelems"] = E / (r[mode]["ms_per_step"] / 1e3) / 1e9
+        words, _ = db.totals()
+        if words != out_total:
+            log(f"{algo}: count-only total {words} differs from {out_total}")
+            sys.exit(3)
+        db.close()
+        log(f"{algo}: {r}")
+        return r
+
+    head = time_batch(args.algo)
+    out_total = head["results"]
+    step_ms = head["out"]["ms_per_step"]
+    kern_ms = head["out"]["kernel_ms"]
     value = E / (step_ms / 1e3) / 1e9
     Bytes = algorithmic_bytes(E, out_total, P)
     achieved = Bytes / (kern_ms / 1e3) / 1e9
     log(f"step {step_ms:.4f} ms, main kernel {kern_ms:.4f} ms, {value:.1f} Gelem/s, {achieved:.0f} GB/s")
 
-    extra = {}
-    # Counter-visitor mode (count-only): same tiles, no output placement
-    for _ in range(2):
-        db.run(stream, count_only=True)
-    torch.cuda.synchronize()
-    db.kernel_time()
-    db.set_timing(True)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        db.run(stream, count_only=True)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    co_ms = ev0.elapsed_time(ev1) / args.steps
-    co_k, co_runs = db.kernel_time()
-    db.set_timing(False)
-    words, _ = db.totals()
-    if words != out_total:
-        log(f"count-only total {words} differs from {out_total}")
-        sys.exit(3)
-    extra["count_only"] = {"ms_per_step": co_ms, "kernel_ms": co_k / max(co_runs, 1), "gelems": E / (co_ms / 1e3) / 1e9}
-    log(f"count-only: {extra['count_only']}")
+    extra = {"count_only": head["count"], "compare": {}}
+    for name in [x for x in args.compare.split(",") if x and x != args.algo]:
+        c = time_batch(name)
+        c["roofline_frac"] = Bytes / (c["out"]["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
+        extra["compare"][name] = c
     cpu = None
     if not args.no_baselines:
         pg = plain_gpu(torch, A, a_off, B, b_off, args.plain_sample, stream)
@@ -481,7 +493,7 @@ def main():
             "larger_log2_range": [6, 14],
             "max_skew": 1024,
             "density": 0.1,
-            "algorithm": "hip_branchless_merge",
+            "algorithm": args.algo,
         },
         "roofline": {
             "bound": "hbm",
@@ -489,9 +501,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(MERGE_KERNEL),
+            "traffic": measured_traffic(DOMINANT_KERNEL[args.algo]),
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
-            "kernel": MERGE_KERNEL,
+            "kernel": DOMINANT_KERNEL[args.algo],
             "kernel_ms": kern_ms,
             "algorithmic_bytes": Bytes,
         },

@@ -170,16 +170,20 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
             const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
             const uint32_t boff = 4u * ((shift_a + n_a + 3) / 4);
+            // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks);
+            // the staging loads and the segment table's loads are in flight together
+            constexpr int QRUN = (U + 12 + 4 * NT - 1) / (4 * NT);
+            TwoRunStage<NT, QRUN> rk, rs;
             if (!bad) {
-                stage_range<NT>(in.A, in.a_pool, ga0, n_a, sm.key);
-                stage_range<NT>(in.B, in.b_pool, gb0, n_b, sm.key + boff);
-                if constexpr (BSR) {
-                    stage_range<NT>(in.As, in.a_pool, ga0, n_a, sm.st);
-                    stage_range<NT>(in.Bs, in.b_pool, gb0, n_b, sm.st + boff);
-                }
+                rk.issue(in.A, in.a_pool, ga0, n_a, in.B, in.b_pool, gb0, n_b);
+                if constexpr (BSR) rs.issue(in.As, in.a_pool, ga0, n_a, in.Bs, in.b_pool, gb0, n_b);
+            }
+            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            if (!bad) {
+                rk.store(in.A, in.a_pool, in.B, in.b_pool, sm.key);
+                if constexpr (BSR) rs.store(in.As, in.a_pool, in.Bs, in.b_pool, sm.st);
             }
             if (tid < nseg) {
-                w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
                 sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
                 sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);
             }
@@ -213,11 +217,13 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             }
             if (tid == 0) sm.win_c[2 * nseg] = totA + totB;
             __syncthreads();
+            constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
             if (!bad) {
-                stage_windows<NT>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg, sm.key);
+                stage_windows<NT, QWIN>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg,
+                                        sm.key);
                 if constexpr (BSR)
-                    stage_windows<NT>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg,
-                                      sm.st);
+                    stage_windows<NT, QWIN>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src,
+                                            2 * nseg, sm.st);
             }
         }
         if (bad) {
@@ -326,8 +332,8 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         // read per step (the side that advanced).
         auto step = [&](uint32_t k) {
             const bool aok = pa < ea, bok = pb < eb;
-            const bool takeA = aok && (!bok || a <= b);
-            bool hit = takeA && pb < ebp && a == b;
+            const bool takeA = aok & ((!bok) | (a <= b));
+            bool hit = takeA & (pb < ebp) & (a == b);
             if constexpr (BSR) {
                 uint32_t stv = 0;
                 if (hit) stv = sm.st[pa] & sm.st[pb];
@@ -337,14 +343,39 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             }
             smask |= hit ? (1u << k) : 0u;
             tmask |= takeA ? (1u << k) : 0u;
-            pa += takeA ? 1u : 0u;
-            pb += (!takeA && bok) ? 1u : 0u;
+            pa += takeA;
+            pb += (!takeA) & bok;
+            const uint32_t v = sm.key[takeA ? pa : pb];
+            a = takeA ? v : a;
+            b = takeA ? b : v;
+        };
+        // Fast path: K merge steps stay inside both windows for every lane of the wave (no early
+        // end, no pair change, >= K elements left on each side), so neither side can run out and
+        // the boundary element cannot be reached: take A iff a <= b, and a == b is a hit.
+        auto step_fast = [&](uint32_t k) {
+            const bool takeA = a <= b;
+            bool hit = a == b;
+            if constexpr (BSR) {
+                uint32_t stv = 0;
+                if (hit) stv = sm.st[pa] & sm.st[pb];
+                hit = hit && stv != 0;
+                svals[k] = stv;
+                nelems += __popc(stv);
+            }
+            smask |= hit ? (1u << k) : 0u;
+            tmask |= takeA ? (1u << k) : 0u;
+            pa += takeA;
+            pb += !takeA;
             const uint32_t v = sm.key[takeA ? pa : pb];
             a = takeA ? v : a;
             b = takeA ? b : v;
         };
         const bool uniform = ballot(kmax != (uint32_t)K || kt != (uint32_t)K) == 0;
-        if (uniform) {
+        const bool fast = uniform && ballot(pa + K > ea || pb + K > eb) == 0;
+        if (fast) {
+#pragma unroll
+            for (int k = 0; k < K; k++) step_fast((uint32_t)k);
+        } else if (uniform) {
             // no lane of this wave ends its tile early or crosses into another pair
 #pragma unroll
             for (int k = 0; k < K; k++) step((uint32_t)k);

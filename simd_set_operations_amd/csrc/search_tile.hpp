@@ -129,9 +129,10 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             ulen = 0;
             if (tid == 0 && out.error) atomicOr(out.error, 2u);
         } else {
-            stage_windows<NT>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey);
+            constexpr int QP = (Smem::PKEYS + 4 * NT - 1) / (4 * NT);
+            stage_windows<NT, QP>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey);
             if constexpr (BSR)
-                stage_windows<NT>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst);
+                stage_windows<NT, QP>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst);
         }
         __syncthreads();
 
@@ -190,10 +191,11 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             if (tid == 0 && nst == 0) sm.win_c[0] = 0;
             __syncthreads();
             if (nst) {
-                stage_windows<NT>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst, sm.lkey);
+                constexpr int QL = (Smem::LKEYS + 4 * NT - 1) / (4 * NT);
+                stage_windows<NT, QL>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst, sm.lkey);
                 if constexpr (BSR)
-                    stage_windows<NT>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst,
-                                      sm.lst);
+                    stage_windows<NT, QL>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst,
+                                          sm.lst);
             }
             __syncthreads();
         }

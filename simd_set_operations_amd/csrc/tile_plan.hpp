@@ -419,62 +419,106 @@ __device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan
     return w;
 }
 
-// Copies `n` elements starting at global element index g0 of `src` (pool length `pool`) into LDS
-// at dst[shift ...], where shift = g0 & 3: 16-byte loads of the aligned chunks covering the range,
-// one chunk per thread per iteration (coalesced), the pool's last partial chunk element-wise.
-template <int NT>
-__device__ __forceinline__ void stage_range(const uint32_t *__restrict__ src, uint64_t pool, uint64_t g0, uint32_t n,
-                                            uint32_t *dst)
+// One 16-byte chunk of a pool at element index e (16-byte aligned), guarding the pool end.
+__device__ __forceinline__ uint4 load_chunk(const uint32_t *__restrict__ src, uint64_t pool, uint64_t e)
 {
-    const uint64_t base = g0 & ~3ull;
-    const uint32_t span = (uint32_t)(g0 - base) + n;
-    const uint32_t chunks = (span + 3) / 4;
-    for (uint32_t c = threadIdx.x; c < chunks; c += NT) {
-        const uint64_t e = base + 4ull * c;
-        uint4 v;
-        if (e + 4 <= pool) {
-            v = *reinterpret_cast<const uint4 *>(src + e);
-        } else {
-            v.x = (e + 0 < pool) ? src[e + 0] : 0u;
-            v.y = (e + 1 < pool) ? src[e + 1] : 0u;
-            v.z = (e + 2 < pool) ? src[e + 2] : 0u;
-            v.w = (e + 3 < pool) ? src[e + 3] : 0u;
-        }
-        *reinterpret_cast<uint4 *>(dst + 4 * c) = v;
+    uint4 v;
+    if (e + 4 <= pool) {
+        v = *reinterpret_cast<const uint4 *>(src + e);
+    } else {
+        v.x = (e + 0 < pool) ? src[e + 0] : 0u;
+        v.y = (e + 1 < pool) ? src[e + 1] : 0u;
+        v.z = (e + 2 < pool) ? src[e + 2] : 0u;
+        v.w = (e + 3 < pool) ? src[e + 3] : 0u;
     }
+    return v;
 }
+
+// Stages two runs into LDS as whole aligned 16-byte chunks: A[ga0, ga0+nA) at dst[(ga0&3) ...] and
+// B[gb0, gb0+nB) right after the A run's chunks (at dst[4*cA + (gb0&3) ...]).  Chunk c of the
+// combined list goes to dst[4c].  issue() sends every load of the lane (chunks tid, tid+NT, ...)
+// before anything waits on them, store() writes them to LDS: a tile's staging costs one memory
+// round trip, and the caller can issue its other loads in between.  QMAX * NT must cover the
+// chunk count.
+template <int NT, int QMAX>
+struct TwoRunStage {
+    uint4 v[QMAX];
+    uint32_t tail = 0;  // chunks that end past their pool: loaded element-wise in store()
+    uint32_t cA = 0, total = 0;
+    uint64_t abase = 0, bbase = 0;
+
+    __device__ __forceinline__ void issue(const uint32_t *__restrict__ A, uint64_t apool, uint64_t ga0, uint32_t nA,
+                                          const uint32_t *__restrict__ B, uint64_t bpool, uint64_t gb0, uint32_t nB)
+    {
+        abase = ga0 & ~3ull;
+        bbase = gb0 & ~3ull;
+        cA = ((uint32_t)(ga0 - abase) + nA + 3) / 4;
+        total = cA + ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
+#pragma unroll
+        for (int k = 0; k < QMAX; k++) {
+            const uint32_t c = threadIdx.x + (uint32_t)k * NT;
+            const bool isA = c < cA;
+            const uint32_t *src = isA ? A : B;
+            const uint64_t pool = isA ? apool : bpool;
+            const uint64_t e = isA ? abase + 4ull * c : bbase + 4ull * (c - cA);
+            const bool full = c < total && e + 4 <= pool;
+            tail |= (c < total && !full) ? (1u << k) : 0u;
+            if (full) v[k] = *reinterpret_cast<const uint4 *>(src + e);
+        }
+    }
+
+    __device__ __forceinline__ void store(const uint32_t *__restrict__ A, uint64_t apool,
+                                          const uint32_t *__restrict__ B, uint64_t bpool, uint32_t *dst)
+    {
+        if (tail) {
+#pragma unroll
+            for (int k = 0; k < QMAX; k++) {
+                if (tail & (1u << k)) {
+                    const uint32_t c = threadIdx.x + (uint32_t)k * NT;
+                    const bool isA = c < cA;
+                    v[k] = load_chunk(isA ? A : B, isA ? apool : bpool,
+                                      isA ? abase + 4ull * c : bbase + 4ull * (c - cA));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QMAX; k++) {
+            const uint32_t c = threadIdx.x + (uint32_t)k * NT;
+            if (c < total) *reinterpret_cast<uint4 *>(dst + 4 * c) = v[k];
+        }
+    }
+};
 
 // Gather staging: nwin windows, window w = src_w[g0_w .. g0_w + n_w), each placed in LDS at
 // dst + 4*cpfx[w] + (g0_w & 3) as whole aligned 16-byte chunks.  cpfx is the exclusive prefix of
-// the windows' chunk counts (cpfx[nwin] = total).  Each thread takes chunks c, c + NT, ... and
-// finds the owning window by binary search over cpfx.
-template <int NT>
+// the windows' chunk counts (cpfx[nwin] = total).  Each thread takes chunks c, c + NT, ... (at most
+// QMAX of them), finds each chunk's window by binary search over cpfx, and issues all its loads
+// before storing.
+template <int NT, int QMAX>
 __device__ __forceinline__ void stage_windows(const uint32_t *__restrict__ srcA, uint64_t poolA,
                                               const uint32_t *__restrict__ srcB, uint64_t poolB,
                                               const uint32_t *cpfx, const uint64_t *gbase, const uint32_t *wsrc,
                                               uint32_t nwin, uint32_t *dst)
 {
     const uint32_t total = cpfx[nwin];
-    for (uint32_t c = threadIdx.x; c < total; c += NT) {
-        uint32_t lo = 0, hi = nwin;  // largest w with cpfx[w] <= c
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cpfx[mid] <= c) lo = mid; else hi = mid;
+    uint4 v[QMAX];
+#pragma unroll
+    for (int k = 0; k < QMAX; k++) {
+        const uint32_t c = threadIdx.x + (uint32_t)k * NT;
+        if (c < total) {
+            uint32_t lo = 0, hi = nwin;  // largest w with cpfx[w] <= c
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (cpfx[mid] <= c) lo = mid; else hi = mid;
+            }
+            const bool fromB = wsrc[lo] != 0;
+            v[k] = load_chunk(fromB ? srcB : srcA, fromB ? poolB : poolA, gbase[lo] + 4ull * (c - cpfx[lo]));
         }
-        const bool fromB = wsrc[lo] != 0;
-        const uint32_t *__restrict__ src = fromB ? srcB : srcA;
-        const uint64_t pool = fromB ? poolB : poolA;
-        const uint64_t e = gbase[lo] + 4ull * (c - cpfx[lo]);
-        uint4 v;
-        if (e + 4 <= pool) {
-            v = *reinterpret_cast<const uint4 *>(src + e);
-        } else {
-            v.x = (e + 0 < pool) ? src[e + 0] : 0u;
-            v.y = (e + 1 < pool) ? src[e + 1] : 0u;
-            v.z = (e + 2 < pool) ? src[e + 2] : 0u;
-            v.w = (e + 3 < pool) ? src[e + 3] : 0u;
-        }
-        *reinterpret_cast<uint4 *>(dst + 4 * c) = v;
+    }
+#pragma unroll
+    for (int k = 0; k < QMAX; k++) {
+        const uint32_t c = threadIdx.x + (uint32_t)k * NT;
+        if (c < total) *reinterpret_cast<uint4 *>(dst + 4 * c) = v[k];
     }
 }
 
