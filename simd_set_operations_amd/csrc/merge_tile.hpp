@@ -125,123 +125,18 @@ __device__ __forceinline__ uint32_t block_intersect(const uint32_t *key, const u
     return mask;
 }
 
-// Count + stage: one workgroup per tile (tiles tile0 + blockIdx.x), no communication between
-// workgroups.  Each tile's results go to its own staging slot (or nowhere in count-only mode);
-// its counts go to tile_count / tile_elems; per-pair counts and in-tile offsets are recorded for
-// the placement kernel.
-template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP = LP_MERGE>
-__global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
+// Steps 3-6 of a tile whose windows and segment table are in LDS (seg_* arrays, key/st): the
+// lanes' merge (or block policy), the tile scan, staged outputs, tile and per-pair counts.
+template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP>
+__device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR, GATHER, LP != LP_MERGE> &sm,
+                                                   const TilePlan &plan, const TileOut &out, uint32_t t,
+                                                   uint32_t nseg, uint32_t ulen, uint32_t p_first)
 {
-    static_assert(K >= 1 && K <= 31, "lane masks hold K bits");
-    constexpr int U = NT * K;
     constexpr bool BLOCKP = LP != LP_MERGE;
-    using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
-    __shared__ Smem sm;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const uint32_t wave = tid / kWave;
-
     {
-        // one tile per workgroup (a loop over tiles here costs ~20% on the merge kernel)
-        const uint32_t t = tile0 + blockIdx.x;
-        if (t >= plan.ntiles) return;
-        // ---- 1. descriptor and segment windows -------------------------------------------------
-        const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
-        const uint64_t u0 = (uint64_t)t * U;
-        uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
-        uint32_t nseg = p_last - p_first + 1;
-        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG;
-        if (bad) nseg = 0;
-        SegWin w{};
-
-        // ---- 2. stage the windows --------------------------------------------------------------
-        if constexpr (!GATHER) {
-            // CSR: the tile's A run and B run come straight from the split (issued before the
-            // segment table's loads)
-            uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
-            if (!bad) {
-                ga0 = plan.tile_ga[t];
-                ga1 = plan.tile_ga[t + 1];
-                gb0 = plan.tile_gb[t];
-                gb1 = plan.tile_gb[t + 1];
-                bad = ga1 < ga0 || gb1 < gb0 || (ga1 - ga0) + (gb1 - gb0) > (uint64_t)U;
-            }
-            const uint32_t n_a = bad ? 0u : (uint32_t)(ga1 - ga0);
-            const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
-            const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
-            const uint32_t boff = 4u * ((shift_a + n_a + 3) / 4);
-            // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks);
-            // the staging loads and the segment table's loads are in flight together
-            constexpr int QRUN = (U + 12 + 4 * NT - 1) / (4 * NT);
-            TwoRunStage<NT, QRUN> rk, rs;
-            if (!bad) {
-                rk.issue(in.A, in.a_pool, ga0, n_a, in.B, in.b_pool, gb0, n_b);
-                if constexpr (BSR) rs.issue(in.As, in.a_pool, ga0, n_a, in.Bs, in.b_pool, gb0, n_b);
-            }
-            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
-            if (!bad) {
-                rk.store(in.A, in.a_pool, in.B, in.b_pool, sm.key);
-                if constexpr (BSR) rs.store(in.As, in.a_pool, in.Bs, in.b_pool, sm.st);
-            }
-            if (tid < nseg) {
-                sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
-                sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);
-            }
-        } else {
-            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
-            uint32_t ca = 0, cb = 0;
-            uint64_t ga = 0, gb = 0;
-            if (tid < nseg) {
-                ga = w.a + w.i0;
-                gb = w.b + w.j0;
-                const uint32_t nwa = (uint32_t)(w.i1 - w.i0);
-                const uint32_t nwb = (uint32_t)(w.j1 - w.j0) + (w.j1 < w.nb ? 1u : 0u);
-                ca = nwa ? ((uint32_t)(ga & 3) + nwa + 3) / 4 : 0u;
-                cb = nwb ? ((uint32_t)(gb & 3) + nwb + 3) / 4 : 0u;
-            }
-            // per-window chunk counts fit 16 bits (a tile holds <= U + 8*MAXSEG words)
-            uint32_t tot = 0;
-            const uint32_t pfx = block_exclusive_scan<NT>(ca | (cb << 16), sm.scan_scratch, &tot);
-            const uint32_t totA = tot & 0xFFFFu, totB = tot >> 16;
-            if (4u * (totA + totB) + 8u > (uint32_t)Smem::KEYS) bad = true;  // not reachable by the unit bound
-            if (tid < nseg) {
-                const uint32_t pa_ = pfx & 0xFFFFu, pb_ = totA + (pfx >> 16);
-                sm.win_c[tid] = pa_;
-                sm.win_c[nseg + tid] = pb_;
-                sm.win_g[tid] = ga & ~3ull;
-                sm.win_g[nseg + tid] = gb & ~3ull;
-                sm.win_src[tid] = 0;
-                sm.win_src[nseg + tid] = 1;
-                sm.seg_la[tid] = 4u * pa_ + (uint32_t)(ga & 3);
-                sm.seg_lb[tid] = 4u * pb_ + (uint32_t)(gb & 3);
-            }
-            if (tid == 0) sm.win_c[2 * nseg] = totA + totB;
-            __syncthreads();
-            constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
-            if (!bad) {
-                stage_windows<NT, QWIN>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg,
-                                        sm.key);
-                if constexpr (BSR)
-                    stage_windows<NT, QWIN>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src,
-                                            2 * nseg, sm.st);
-            }
-        }
-        if (bad) {
-            nseg = 0;
-            ulen = 0;
-            if (tid == 0 && out.error) atomicOr(out.error, 2u);
-        }
-        if (tid < nseg) {
-            const uint32_t wb = (uint32_t)(w.j1 - w.j0);
-            sm.seg_u[tid] = w.u;
-            sm.seg_flags[tid] = w.flags;
-            sm.seg_wa[tid] = (uint32_t)(w.i1 - w.i0);
-            sm.seg_wb[tid] = wb;
-            sm.seg_nb[tid] = wb + (w.j1 < w.nb ? 1u : 0u);
-        }
-        if (tid == 0) sm.seg_u[nseg] = ulen;
-        __syncthreads();
-
         // ---- 3. per-lane merge ---------------------------------------------------------------------
         // smask: bit k set when step k emitted; tmask: bit k set when step k consumed an A element.
         // The emitted values are A elements, re-read from LDS at write time: the A offset of the
@@ -491,6 +386,125 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                 else if (c) atomicAdd(&out.counts[p], c);
             }
         }
+    }
+}
+
+// Count + stage: one workgroup per tile (tiles tile0 + blockIdx.x), no communication between
+// workgroups.  Each tile's results go to its own staging slot (or nowhere in count-only mode);
+// its counts go to tile_count / tile_elems; per-pair counts and in-tile offsets are recorded for
+// the placement kernel.
+template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP = LP_MERGE>
+__global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
+{
+    static_assert(K >= 1 && K <= 32, "lane masks hold K bits");
+    constexpr int U = NT * K;
+    constexpr bool BLOCKP = LP != LP_MERGE;
+    using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
+    __shared__ Smem sm;
+    const uint32_t tid = threadIdx.x;
+
+    {
+        // one tile per workgroup (a loop over tiles here costs ~20% on the merge kernel)
+        const uint32_t t = tile0 + blockIdx.x;
+        if (t >= plan.ntiles) return;
+        // ---- 1. descriptor and segment windows -------------------------------------------------
+        const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
+        const uint64_t u0 = (uint64_t)t * U;
+        uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
+        uint32_t nseg = p_last - p_first + 1;
+        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG;
+        if (bad) nseg = 0;
+        SegWin w{};
+
+        // ---- 2. stage the windows --------------------------------------------------------------
+        if constexpr (!GATHER) {
+            // CSR: the tile's A run and B run come straight from the split (issued before the
+            // segment table's loads)
+            uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
+            if (!bad) {
+                ga0 = plan.tile_ga[t];
+                ga1 = plan.tile_ga[t + 1];
+                gb0 = plan.tile_gb[t];
+                gb1 = plan.tile_gb[t + 1];
+                bad = ga1 < ga0 || gb1 < gb0 || (ga1 - ga0) + (gb1 - gb0) > (uint64_t)U;
+            }
+            const uint32_t n_a = bad ? 0u : (uint32_t)(ga1 - ga0);
+            const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
+            const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
+            const uint32_t boff = 4u * ((shift_a + n_a + 3) / 4);
+            // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks);
+            // the staging loads and the segment table's loads are in flight together
+            constexpr int QRUN = (U + 12 + 4 * NT - 1) / (4 * NT);
+            TwoRunStage<NT, QRUN> rk, rs;
+            if (!bad) {
+                rk.issue(in.A, in.a_pool, ga0, n_a, in.B, in.b_pool, gb0, n_b);
+                if constexpr (BSR) rs.issue(in.As, in.a_pool, ga0, n_a, in.Bs, in.b_pool, gb0, n_b);
+            }
+            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            if (!bad) {
+                rk.store(in.A, in.a_pool, in.B, in.b_pool, sm.key);
+                if constexpr (BSR) rs.store(in.As, in.a_pool, in.Bs, in.b_pool, sm.st);
+            }
+            if (tid < nseg) {
+                sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
+                sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);
+            }
+        } else {
+            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            uint32_t ca = 0, cb = 0;
+            uint64_t ga = 0, gb = 0;
+            if (tid < nseg) {
+                ga = w.a + w.i0;
+                gb = w.b + w.j0;
+                const uint32_t nwa = (uint32_t)(w.i1 - w.i0);
+                const uint32_t nwb = (uint32_t)(w.j1 - w.j0) + (w.j1 < w.nb ? 1u : 0u);
+                ca = nwa ? ((uint32_t)(ga & 3) + nwa + 3) / 4 : 0u;
+                cb = nwb ? ((uint32_t)(gb & 3) + nwb + 3) / 4 : 0u;
+            }
+            // per-window chunk counts fit 16 bits (a tile holds <= U + 8*MAXSEG words)
+            uint32_t tot = 0;
+            const uint32_t pfx = block_exclusive_scan<NT>(ca | (cb << 16), sm.scan_scratch, &tot);
+            const uint32_t totA = tot & 0xFFFFu, totB = tot >> 16;
+            if (4u * (totA + totB) + 8u > (uint32_t)Smem::KEYS) bad = true;  // not reachable by the unit bound
+            if (tid < nseg) {
+                const uint32_t pa_ = pfx & 0xFFFFu, pb_ = totA + (pfx >> 16);
+                sm.win_c[tid] = pa_;
+                sm.win_c[nseg + tid] = pb_;
+                sm.win_g[tid] = ga & ~3ull;
+                sm.win_g[nseg + tid] = gb & ~3ull;
+                sm.win_src[tid] = 0;
+                sm.win_src[nseg + tid] = 1;
+                sm.seg_la[tid] = 4u * pa_ + (uint32_t)(ga & 3);
+                sm.seg_lb[tid] = 4u * pb_ + (uint32_t)(gb & 3);
+            }
+            if (tid == 0) sm.win_c[2 * nseg] = totA + totB;
+            __syncthreads();
+            constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
+            if (!bad) {
+                stage_windows<NT, QWIN>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg,
+                                        sm.key);
+                if constexpr (BSR)
+                    stage_windows<NT, QWIN>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src,
+                                            2 * nseg, sm.st);
+            }
+        }
+        if (bad) {
+            nseg = 0;
+            ulen = 0;
+            if (tid == 0 && out.error) atomicOr(out.error, 2u);
+        }
+        if (tid < nseg) {
+            const uint32_t wb = (uint32_t)(w.j1 - w.j0);
+            sm.seg_u[tid] = w.u;
+            sm.seg_flags[tid] = w.flags;
+            sm.seg_wa[tid] = (uint32_t)(w.i1 - w.i0);
+            sm.seg_wb[tid] = wb;
+            sm.seg_nb[tid] = wb + (w.j1 < w.nb ? 1u : 0u);
+        }
+        if (tid == 0) sm.seg_u[nseg] = ulen;
+        __syncthreads();
+
+        merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first);
     }
 }
 

@@ -49,10 +49,11 @@ int fail(int code, const char *fmt, ...)
 // ---------------------------------------------------------------------------------------------
 // kernel configuration
 // ---------------------------------------------------------------------------------------------
-// merge policy: 256 lanes x 16 merge-path steps per tile
-constexpr int kNT = 256;
+// merge policy: kNTM lanes x kK merge-path steps per tile
+constexpr int kNT = 256;  // workgroup size of the search tile
+constexpr int kNTM = 256;
 constexpr int kK = 16;
-constexpr int kU = kNT * kK;
+constexpr int kU = kNTM * kK;
 constexpr int kMaxSeg = 130;
 // every pair owns at least c0 units so a tile touches at most kMaxSeg pairs
 constexpr uint32_t kC0 = (kU + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
@@ -169,6 +170,18 @@ struct setops_hip_batch {
     unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
     hipStream_t last_stream = nullptr;
     bool timing = false;
+    // captured launch sequence (batch_launch)
+    bool use_graph = true;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    struct GraphKeyT {
+        void *out, *out_s;
+        uint64_t cap;
+        void *out_off, *counts, *totals;
+        PairBatch in;
+        TilePlan plan;
+    } graph_key{}, seen_key{};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;  // pending timed runs
     std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
 };
@@ -343,8 +356,8 @@ template <bool BSR, bool CO, bool GA, int LP>
 void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((merge_tile_kernel<kNT, kK, kMaxSeg, BSR, CO, GA, LP>),
-                           dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
+        hipLaunchKernelGGL((merge_tile_kernel<kNTM, kK, kMaxSeg, BSR, CO, GA, LP>),
+                           dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNTM), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
 }
 
@@ -378,6 +391,65 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
     }
 }
 
+// The launch sequence of one run (no allocation, no synchronisation): plan, split, tile kernel,
+// scan, placement.  `ev` (may be null) receives events around the tile kernel.
+int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipStream_t stream,
+                  std::pair<hipEvent_t, hipEvent_t> *ev)
+{
+    // plan: unit prefix and tile maps, then tile starts
+    if (closed_form(h)) {
+        hipLaunchKernelGGL(plan_csr_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
+        HIP_TRY(hipGetLastError());
+    } else {
+        int rc = launch_units_scan(h, out, stream);
+        if (rc != SETOPS_OK) return rc;
+    }
+    hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in, h->plan);
+    HIP_TRY(hipGetLastError());
+    // count (and stage)
+    if (ev) HIP_TRY(hipEventRecord(ev->first, stream));
+    if (h->bsr) {
+        if (count_only) launch_tiles_b<true, true>(h, out, stream);
+        else launch_tiles_b<true, false>(h, out, stream);
+    } else {
+        if (count_only) launch_tiles_b<false, true>(h, out, stream);
+        else launch_tiles_b<false, false>(h, out, stream);
+    }
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev->second, stream));
+    // scan: group sums, one-block scan of the group sums (writes the totals), per-group scan
+    const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
+    hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                       h->plan.gsum2, (uint64_t)ngroups, 1, out.totals ? out.totals : h->d_tot);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, out);
+    HIP_TRY(hipGetLastError());
+    // write
+    if (!count_only || out.out_off) {
+        const uint32_t gp = grid_for(h->plan.ntiles, 16);  // 16 tiles per workgroup (16 lanes each)
+        if (h->bsr)
+            hipLaunchKernelGGL((place_tiles_kernel<256, true, 4>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
+        else
+            hipLaunchKernelGGL((place_tiles_kernel<256, false, 4>), dim3(gp), dim3(256), 0, stream, h->in, h->plan,
+                               out);
+        HIP_TRY(hipGetLastError());
+    }
+    return SETOPS_OK;
+}
+
+void graph_drop(setops_hip_batch *h)
+{
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+}
+
+// A run: eager when timed (events around the tile kernel), otherwise the launch sequence is
+// captured once into a HIP graph (per handle, per set of output pointers and plan) and replayed
+// with one launch.  The sequence records kernels only (no memset or copy nodes).
 int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64_t cap, uint64_t *d_out_off,
                  uint32_t *d_counts, unsigned long long *d_totals, hipStream_t stream)
 {
@@ -398,59 +470,57 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;
     }
-    // plan: unit prefix and tile maps, then tile starts
-    if (closed_form(h)) {
-        hipLaunchKernelGGL(plan_csr_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
-        HIP_TRY(hipGetLastError());
-    } else {
-        int rc = launch_units_scan(h, out, stream);
-        if (rc != SETOPS_OK) return rc;
-    }
-    hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in, h->plan);
-    HIP_TRY(hipGetLastError());
-    // count (and stage)
-    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-    if (h->timing) {
-        if (!h->spare.empty()) {
-            ev = h->spare.back();
-            h->spare.pop_back();
-        } else {
-            HIP_TRY(hipEventCreate(&ev.first));
-            HIP_TRY(hipEventCreate(&ev.second));
-        }
-        HIP_TRY(hipEventRecord(ev.first, stream));
-    }
-    if (h->bsr) {
-        if (count_only) launch_tiles_b<true, true>(h, out, stream);
-        else launch_tiles_b<true, false>(h, out, stream);
-    } else {
-        if (count_only) launch_tiles_b<false, true>(h, out, stream);
-        else launch_tiles_b<false, false>(h, out, stream);
-    }
-    HIP_TRY(hipGetLastError());
-    if (h->timing) {
-        HIP_TRY(hipEventRecord(ev.second, stream));
-        h->events.push_back(ev);
-    }
-    // scan: group sums, one-block scan of the group sums (writes the totals), per-group scan
-    const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
-    hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
-                       h->plan.gsum2, (uint64_t)ngroups, 1, d_totals ? d_totals : h->d_tot);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, out);
-    HIP_TRY(hipGetLastError());
-    // write
-    if (!count_only || d_out_off) {
-        const uint32_t gp = grid_for(h->plan.ntiles, 4);  // 4 tiles (waves) per workgroup
-        if (h->bsr)
-            hipLaunchKernelGGL((place_tiles_kernel<256, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
-        else
-            hipLaunchKernelGGL((place_tiles_kernel<256, false>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
-        HIP_TRY(hipGetLastError());
-    }
     h->last_stream = stream;
+    if (h->timing || !h->use_graph) {
+        std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+        if (h->timing) {
+            if (!h->spare.empty()) {
+                ev = h->spare.back();
+                h->spare.pop_back();
+            } else {
+                HIP_TRY(hipEventCreate(&ev.first));
+                HIP_TRY(hipEventCreate(&ev.second));
+            }
+        }
+        int rc = batch_enqueue(h, out, count_only, stream, h->timing ? &ev : nullptr);
+        if (h->timing) h->events.push_back(ev);
+        return rc;
+    }
+    // the captured sequence depends on the outputs and on everything the plan holds
+    setops_hip_batch::GraphKeyT key;
+    std::memset(static_cast<void *>(&key), 0, sizeof key);
+    key.out = d_out;
+    key.out_s = d_out_s;
+    key.cap = out.cap;
+    key.out_off = d_out_off;
+    key.counts = d_counts;
+    key.totals = d_totals;
+    key.in = h->in;
+    key.plan = h->plan;
+    const bool cached = h->graph_exec && std::memcmp(&key, &h->graph_key, sizeof key) == 0;
+    if (!cached && std::memcmp(&key, &h->seen_key, sizeof key) != 0) {
+        // first run with these arguments: eager (a run whose arguments change every time, like a
+        // rebound serving batch or a single pair, never pays for a capture)
+        h->seen_key = key;
+        return batch_enqueue(h, out, count_only, stream, nullptr);
+    }
+    if (!cached) {
+        graph_drop(h);
+        if (!h->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue(h, out, count_only, h->cap_stream, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->cap_stream, &g);
+        if (rc != SETOPS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(SETOPS_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        h->graph = g;
+        HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+        h->graph_key = key;
+    }
+    HIP_TRY(hipGraphLaunch(h->graph_exec, stream));
     return SETOPS_OK;
 }
 
@@ -524,6 +594,8 @@ void batch_free(setops_hip_batch *h)
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    graph_drop(h);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     if (h->ws) (void)hipFree(h->ws);
     if (h->stage) (void)hipFree(h->stage);
     delete h;

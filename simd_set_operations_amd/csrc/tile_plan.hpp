@@ -250,6 +250,7 @@ __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan p
 
 // Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
 // A index i and B index d - i with every A[x < i] <= B[d-1-...]), or the probe index (search).
+// One thread per tile (a cooperative search costs more loads in total than it saves latency).
 __global__ void plan_split_kernel(PairBatch in, TilePlan plan)
 {
     const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
@@ -344,14 +345,16 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
 }
 
 // Write: moves each tile's staged results to their final position and turns in-tile pair offsets
-// into output offsets.  One wave per tile, NT/64 tiles per workgroup, grid-stride.
-template <int NT, bool BSR>
+// into output offsets.  A group of 16 lanes per tile (4 tiles per wave), grid-stride.  Staging
+// slots are 16-byte aligned: a lane reads four results with one load (up to QP loads, all issued
+// before any store) and writes them to consecutive output words (any alignment).
+template <int NT, bool BSR, int QP>
 __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out)
 {
-    const uint32_t l = lane_id();
-    const uint32_t wpb = NT / kWave;
-    for (uint64_t t = (uint64_t)blockIdx.x * wpb + threadIdx.x / kWave; t < plan.ntiles;
-         t += (uint64_t)gridDim.x * wpb) {
+    constexpr uint32_t G = 16;
+    const uint32_t g = lane_id() % G;
+    const uint64_t gstride = (uint64_t)gridDim.x * NT / G;
+    for (uint64_t t = ((uint64_t)blockIdx.x * NT + threadIdx.x) / G; t < plan.ntiles; t += gstride) {
         const uint64_t base = plan.tile_off[t];
         const uint32_t n = plan.tile_count[t];
         if (out.out) {
@@ -359,17 +362,48 @@ __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan 
             const uint32_t *ss = sk + plan.stage_words / 2;
             uint64_t lim = n;
             if (base + n > out.cap) lim = (base < out.cap) ? out.cap - base : 0;
-            for (uint32_t x = l; x < lim; x += kWave) {
-                out.out[base + x] = sk[x];
-                if constexpr (BSR) out.out_s[base + x] = ss[x];
+            for (uint32_t x0 = 0; x0 < lim; x0 += 4 * G * QP) {
+                uint4 v[QP], w[BSR ? QP : 1];
+#pragma unroll
+                for (int k = 0; k < QP; k++) {
+                    const uint32_t x = x0 + 4 * (g + G * k);
+                    if (x < lim) {
+                        v[k] = *reinterpret_cast<const uint4 *>(sk + x);
+                        if constexpr (BSR) w[k] = *reinterpret_cast<const uint4 *>(ss + x);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < QP; k++) {
+                    const uint32_t x = x0 + 4 * (g + G * k);
+                    if (x < lim) {
+                        // global memory takes 16-byte stores at 4-byte alignment: one store for a
+                        // full group of four, single words for the tail
+                        const uint64_t m = lim - x;
+                        uint32_t *o = out.out + base + x;
+                        uint32_t *os = BSR ? out.out_s + base + x : nullptr;
+                        if (m >= 4) {
+                            *reinterpret_cast<uint4 *>(o) = v[k];
+                            if constexpr (BSR) *reinterpret_cast<uint4 *>(os) = w[k];
+                        } else {
+                            o[0] = v[k].x;
+                            if (m > 1) o[1] = v[k].y;
+                            if (m > 2) o[2] = v[k].z;
+                            if constexpr (BSR) {
+                                os[0] = w[k].x;
+                                if (m > 1) os[1] = w[k].y;
+                                if (m > 2) os[2] = w[k].z;
+                            }
+                        }
+                    }
+                }
             }
         }
         if (out.out_off) {
             const uint64_t u0 = t * plan.U;
             const uint32_t pf = plan.tile_first[t], pl = plan.tile_last[t];
-            for (uint32_t p = pf + l; p <= pl; p += kWave)
+            for (uint32_t p = pf + g; p <= pl; p += G)
                 if (plan.uoff[p] >= u0) out.out_off[p] = base + plan.pair_local[p];
-            if (t == plan.ntiles - 1 && l == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
+            if (t == plan.ntiles - 1 && g == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
         }
     }
 }
@@ -388,10 +422,30 @@ struct SegWin {
     uint32_t flags;    // bit0: pair starts in this tile, bit1: pair ends in this tile
 };
 
+// Where a tile starts and where the next one starts (both inside their first pairs), as the
+// segment table needs them.
+struct TileEdges {
+    uint32_t sa, sb;         // tile_sa[t], tile_sb[t]
+    uint32_t next_first;     // tile_first[t+1] (~0 for the last tile)
+    uint32_t next_sa, next_sb;
+};
+
+__device__ __forceinline__ TileEdges tile_edges(const TilePlan &plan, uint32_t t)
+{
+    TileEdges e;
+    e.sa = plan.tile_sa[t];
+    e.sb = plan.tile_sb[t];
+    const bool more = t + 1 < plan.ntiles;
+    e.next_first = more ? plan.tile_first[t + 1] : ~0u;
+    e.next_sa = more ? plan.tile_sa[t + 1] : 0u;
+    e.next_sb = more ? plan.tile_sb[t + 1] : 0u;
+    return e;
+}
+
 // Window of segment s of tile t (pairs p_first .. p_last); the last segment ends where the next
 // tile starts if that tile starts in the same pair, else at the pair's end.
 __device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan &plan, uint32_t t, uint32_t p_first,
-                                             uint32_t nseg, uint32_t s, uint32_t ulen)
+                                             uint32_t nseg, uint32_t s, uint32_t ulen, const TileEdges &e)
 {
     SegWin w;
     const uint64_t p = (uint64_t)p_first + s;
@@ -402,21 +456,27 @@ __device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan
     w.nb = in.b_end[p] - w.b;
     const uint64_t up = plan.uoff[p], upn = plan.uoff[p + 1];
     const uint64_t u0 = (uint64_t)t * plan.U;
-    w.i0 = (s == 0) ? plan.tile_sa[t] : 0;
-    w.j0 = (s == 0) ? plan.tile_sb[t] : 0;
-    const bool cont = (s == nseg - 1) && (t + 1 < plan.ntiles) && plan.tile_first[t + 1] == (uint32_t)p;
+    w.i0 = (s == 0) ? e.sa : 0;
+    w.j0 = (s == 0) ? e.sb : 0;
+    const bool cont = (s == nseg - 1) && e.next_first == (uint32_t)p;
     if (plan.policy == POLICY_SEARCH) {
         const uint64_t np = w.na < w.nb ? w.na : w.nb;
-        w.i1 = cont ? plan.tile_sa[t + 1] : np;
+        w.i1 = cont ? e.next_sa : np;
         w.j0 = 0;
         w.j1 = 0;
     } else {
-        w.i1 = cont ? plan.tile_sa[t + 1] : w.na;
-        w.j1 = cont ? plan.tile_sb[t + 1] : w.nb;
+        w.i1 = cont ? e.next_sa : w.na;
+        w.j1 = cont ? e.next_sb : w.nb;
     }
     w.u = (up > u0) ? (uint32_t)(up - u0) : 0u;
     w.flags = (up >= u0 ? 1u : 0u) | (upn <= u0 + ulen ? 2u : 0u);
     return w;
+}
+
+__device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan &plan, uint32_t t, uint32_t p_first,
+                                             uint32_t nseg, uint32_t s, uint32_t ulen)
+{
+    return seg_window(in, plan, t, p_first, nseg, s, ulen, tile_edges(plan, t));
 }
 
 // One 16-byte chunk of a pool at element index e (16-byte aligned), guarding the pool end.
@@ -453,17 +513,21 @@ struct TwoRunStage {
         abase = ga0 & ~3ull;
         bbase = gb0 & ~3ull;
         cA = ((uint32_t)(ga0 - abase) + nA + 3) / 4;
-        total = cA + ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
+        const uint32_t cB = ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
+        total = cA + cB;
+        // uniform: each run's first chunk and how many of its chunks lie wholly inside the pool
+        const uint4 *pa = reinterpret_cast<const uint4 *>(A + abase);
+        const uint4 *pb = reinterpret_cast<const uint4 *>(B + bbase);
+        const uint64_t fa = (apool - abase) / 4, fb = (bpool - bbase) / 4;
+        const uint32_t fullA = fa < cA ? (uint32_t)fa : cA, fullB = fb < cB ? (uint32_t)fb : cB;
 #pragma unroll
         for (int k = 0; k < QMAX; k++) {
             const uint32_t c = threadIdx.x + (uint32_t)k * NT;
             const bool isA = c < cA;
-            const uint32_t *src = isA ? A : B;
-            const uint64_t pool = isA ? apool : bpool;
-            const uint64_t e = isA ? abase + 4ull * c : bbase + 4ull * (c - cA);
-            const bool full = c < total && e + 4 <= pool;
+            const uint32_t cl = isA ? c : c - cA;
+            const bool full = c < total && cl < (isA ? fullA : fullB);
             tail |= (c < total && !full) ? (1u << k) : 0u;
-            if (full) v[k] = *reinterpret_cast<const uint4 *>(src + e);
+            if (full) v[k] = (isA ? pa : pb)[cl];
         }
     }
 

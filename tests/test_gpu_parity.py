@@ -464,3 +464,37 @@ def test_served_queries_alone_and_in_the_mix():
         _, r1, _ = eng.serve(ids[q:q + 1], ks[q:q + 1], np.zeros(1), keep_results=True)
         assert np.array_equal(r1[0], res[q]), q
     eng.close()
+
+
+def test_batch_graph_replays_match():
+    """Repeated runs with the same arguments replay a captured graph: every replay, with other
+    device work and synchronisations in between, gives the oracle's results (and the totals)."""
+    import torch
+
+    from simd_set_operations_amd.batch import DeviceBatch
+
+    rng = np.random.Generator(np.random.PCG64(101))
+    As = [sorted_set(rng, int(rng.integers(0, 3000)), 8000) for _ in range(700)]
+    Bs = [sorted_set(rng, int(rng.integers(0, 3000)), 8000) for _ in range(700)]
+    A, a_off = _csr(As)
+    B, b_off = _csr(Bs)
+    counts, off, data = _batch_expect(A, a_off, B, b_off)
+    announce("graph replays P=700")
+    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
+    scratch = torch.zeros(1 << 20, device="cuda")
+    for rep in range(6):
+        db.d_out.fill_(-1)
+        db.d_counts.fill_(-1)
+        db.run()
+        scratch.add_(1.0)  # other work on the same stream between replays
+        torch.cuda.synchronize()
+        g_off, g_cnt, g_data = db.results()
+        assert np.array_equal(g_cnt, counts), rep
+        assert np.array_equal(g_off, off), rep
+        assert np.array_equal(g_data, data), rep
+        assert db.totals()[0] == int(counts.sum()), rep
+    for rep in range(3):
+        db.run(count_only=True)
+        torch.cuda.synchronize()
+        assert db.totals()[0] == int(counts.sum()), rep
+    db.close()
