@@ -76,6 +76,7 @@ HIP_SIGNATURES = {
         _c.c_int,
         [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp, _c.c_uint64],
     ),
+    "setops_hip_batch_index_b": (_c.c_int, [_vp]),
     "setops_hip_batch_rebind": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp]),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),

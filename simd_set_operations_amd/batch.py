@@ -117,6 +117,13 @@ class DeviceBatch(_Batch):
             )
         )
 
+    def index_b(self):
+        """Index the B pool (every 32nd element) for the search-policy entries: larger sets that
+        are searched in global memory then cost about one 128-byte line per probe.  For resident
+        B pools shared by many pairs; the pool must not change afterwards where pairs read it."""
+        _native.check(_native.hip().setops_hip_batch_index_b(self.handle))
+        return self
+
     def check(self):
         _native.check(_native.hip().setops_hip_batch_check(self.handle))
 

@@ -72,6 +72,39 @@ __device__ __forceinline__ uint32_t lower_bound_from(const Arr &arr, uint32_t lo
     return lo;
 }
 
+constexpr uint64_t kNoPos = ~0ull;
+
+// Pool index of x in B[g0, g1), or kNoPos, using idx[k] = B[32k] (every 32nd pool element): a
+// binary search over the index entries inside the range finds the only 32-element line that can
+// hold x, which is compared in full (eight 16-byte loads).  The range is sorted (one set), so the
+// index entries inside it are too.
+__device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, uint64_t pool,
+                                                const uint32_t *__restrict__ idx, uint64_t g0, uint64_t g1, uint32_t x)
+{
+    if (g0 >= g1) return kNoPos;
+    const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;  // entries with 32k in [g0, g1)
+    uint64_t lo = kb, hi = ke;                                 // first entry > x
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (idx[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    if (lo == 0) return kNoPos;  // kb == 0 and x < B[0] = B[g0]
+    const uint64_t base = 32 * (lo - 1);  // the line [base, base+32) holds every candidate
+    uint64_t hit = kNoPos;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint64_t e = base + 4ull * c;
+        const uint4 v = load_chunk(B, pool, e);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t q = e + (uint64_t)j;
+            if (q >= g0 && q < g1 && q < pool && w[j] == x) hit = q;
+        }
+    }
+    return hit;
+}
+
 template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, bool GALLOP>
 __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
@@ -235,6 +268,15 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     found = pos < hi_s && arr(pos) == x;
                     if constexpr (BSR)
                         if (found) lstate = sm.lst[lb + (pos - lo_s)];
+                } else if (in.b_idx && !large_is_a) {
+                    // resident B pool with an index of every 32nd element: binary search the index,
+                    // then compare one 128-byte line of B
+                    const uint64_t g = sm.seg_lbase[s];
+                    const uint64_t q = index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x);
+                    found = q != kNoPos;
+                    pos = found ? (uint32_t)(q - g) : cur;
+                    if constexpr (BSR)
+                        if (found) lstate = in.Bs[q];
                 } else {
                     const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
                     auto arr = [&](uint32_t i) { return L[i]; };

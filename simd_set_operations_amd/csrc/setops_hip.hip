@@ -166,6 +166,7 @@ struct setops_hip_batch {
     uint64_t cap_tiles = 0;
     uint32_t *d_error = nullptr;
     uint32_t *stage = nullptr;  // result staging (separate allocation, see stage_reserve)
+    uint32_t *b_idx = nullptr;  // index of the B pool (setops_hip_batch_index_b)
     uint64_t stage_tiles = 0;
     unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
     hipStream_t last_stream = nullptr;
@@ -598,6 +599,7 @@ void batch_free(setops_hip_batch *h)
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     if (h->ws) (void)hipFree(h->ws);
     if (h->stage) (void)hipFree(h->stage);
+    if (h->b_idx) (void)hipFree(h->b_idx);
     delete h;
 }
 
@@ -861,6 +863,27 @@ int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const ui
     if (!d_a_end || !d_b_end) return fail(SETOPS_ERR_INVALID, "end arrays must not be NULL");
     return batch_create(handle, algo, d_a, d_a_states, a_len, d_a_beg, d_a_end, d_b, d_b_states, b_len, d_b_beg,
                         d_b_end, npairs);
+}
+
+int setops_hip_batch_index_b(setops_hip_batch *handle)
+{
+    if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
+    if (handle->b_idx) return SETOPS_OK;
+    if (merge_like(handle->kind)) return SETOPS_OK;  // only the search policy reads an index
+    const uint64_t m = (handle->in.b_pool + 31) / 32;
+    if (m == 0) return SETOPS_OK;
+    if (hipMalloc(&handle->b_idx, m * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        handle->b_idx = nullptr;
+        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)(m * 4));
+    }
+    hipStream_t s = handle->last_stream ? handle->last_stream : g_ctx.stream;
+    hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
+                       handle->b_idx);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    handle->in.b_idx = handle->b_idx;
+    return SETOPS_OK;
 }
 
 int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, const uint64_t *d_a_end,

@@ -40,7 +40,16 @@ struct PairBatch {
     uint64_t b_pool;
     uint64_t P;
     uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
+    const uint32_t *b_idx;  // optional: b_idx[k] = B[32k] (search policy: resident B pools)
 };
+
+// Index of a resident pool: idx[k] = pool[32k], k < ceil(n / 32).
+__global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx)
+{
+    const uint64_t m = (n + 31) / 32;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+        idx[k] = pool[32 * k];
+}
 
 enum : uint32_t { POLICY_MERGE = 0, POLICY_SEARCH = 1 };
 

@@ -126,9 +126,12 @@ def poisson_arrivals(nq, rate, seed):
 
 
 class Engine:
-    """Continuous-batching SvS engine over a Corpus (one gather batch handle, rebound every step)."""
+    """Continuous-batching SvS engine over a Corpus (one gather batch handle, rebound every step).
 
-    def __init__(self, corpus, algo="hip_galloping", max_pairs=16384):
+    ``index``: index the corpus pool once (every 32nd element), so search-policy entries probe long
+    lists in global memory with about one 128-byte line per lookup."""
+
+    def __init__(self, corpus, algo="hip_galloping", max_pairs=16384, index=True):
         torch = corpus.torch
         self.c = corpus
         self.torch = torch
@@ -152,6 +155,10 @@ class Engine:
             corpus.pool.numel(), one.data_ptr() + 16, one.data_ptr() + 24, 1))
         self.handle = h
         self._keep = one
+        if index:
+            # the B side of every round is a corpus list, never the arena, so an index built
+            # once over the pool stays valid where it is read
+            _native.check(lib.setops_hip_batch_index_b(h))
 
     def close(self):
         if self.handle:

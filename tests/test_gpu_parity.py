@@ -348,8 +348,9 @@ def test_gather_batch_bsr(name):
     gb.close()
 
 
+@pytest.mark.parametrize("indexed", [False, True])
 @pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_branchless_merge"])
-def test_skewed_shared_large_set(name):
+def test_skewed_shared_large_set(name, indexed):
     """The skewed workload's shape at reduced size: one large set shared by every pair (searched
     in global memory when its narrowed window does not fit LDS) against many small sets, each with
     a planted share drawn from the large set."""
@@ -367,15 +368,82 @@ def test_skewed_shared_large_set(name):
         shared.append(hit.size)
     pool, off = _csr(smalls)
     P = len(smalls)
-    announce(f"{name} skewed P={P} |L|={large.size}")
+    announce(f"{name} skewed P={P} |L|={large.size} indexed={indexed}")
     gb = GatherBatch(name, pool, off[:-1], off[1:], large, np.zeros(P, np.uint64),
                      np.full(P, large.size, np.uint64))
+    if indexed:
+        gb.index_b()
     gb.run()
     g_off, g_cnt, g_data = gb.results()
     assert np.array_equal(g_cnt, np.array(shared, np.uint32))
     for p in range(P):
         assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], np.intersect1d(smalls[p], large)), p
     gb.close()
+
+
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_galloping_bsr"])
+def test_indexed_b_pool_ragged_ranges(name):
+    """B-pool index: B ranges that start and end anywhere inside long sets (so index lines straddle
+    range ends and neighbouring sets), empty ranges, and a range that ends at the pool's last
+    element; indexed results equal unindexed ones and the oracle's, on two runs."""
+    from simd_set_operations_amd.batch import GatherBatch
+
+    bsr = name.endswith("_bsr")
+    rng = np.random.Generator(np.random.PCG64(97))
+    sets = [sorted_set(rng, int(rng.integers(4000, 120000)), 1 << 26) for _ in range(24)]
+    sets += [sorted_set(rng, int(rng.integers(0, 40)), 1 << 26) for _ in range(8)]
+    if bsr:
+        bs = [BsrVec.from_sorted(x) for x in sets]
+        pool, off = _csr([b.bases for b in bs])
+        spool, _ = _csr([b.states for b in bs])
+    else:
+        pool, off = _csr(sets)
+    n = np.diff(off)
+    P = 1500
+    ib = rng.integers(0, len(sets), size=P)
+    lo = (n[ib] * rng.random(P) * 0.3).astype(np.uint64)
+    hi = n[ib] - (n[ib] * rng.random(P) * 0.3).astype(np.uint64)
+    hi[::97] = lo[::97]  # empty ranges
+    b_beg, b_end = off[ib] + lo, off[ib] + hi
+    b_beg[5], b_end[5] = off[-2], off[-1]  # the pool's last set, whole
+    # small A sets: half drawn from the B range (hits), half random
+    a_sets = []
+    for p in range(P):
+        rngB = pool[b_beg[p]:b_end[p]]
+        m = int(rng.integers(0, 200))
+        hit = rng.choice(rngB, size=min(m // 2, rngB.size), replace=False) if rngB.size else np.zeros(0, np.uint32)
+        a_sets.append(np.union1d(hit, rng.integers(0, 1 << (21 if bsr else 26), size=m - hit.size)).astype(np.uint32))
+    if bsr:
+        a_st = [rng.integers(1, 1 << 32, size=x.size, dtype=np.uint64).astype(np.uint32) for x in a_sets]
+        apool, aoff = _csr(a_sets)
+        aspool, _ = _csr(a_st)
+    else:
+        apool, aoff = _csr(a_sets)
+    announce(f"{name} indexed B pool={pool.size} P={P}")
+    res = []
+    for indexed in (False, True):
+        kw = {"A_states": aspool, "B_states": spool} if bsr else {}
+        gb = GatherBatch(name, apool, aoff[:-1], aoff[1:], pool, b_beg, b_end, **kw)
+        if indexed:
+            gb.index_b()
+        for rep in range(2):
+            gb.run()
+            res.append(gb.results())
+        gb.close()
+    for r in res[1:]:
+        for x, y in zip(res[0], r):
+            assert np.array_equal(x, y)
+    g_off = res[0][0]
+    for p in range(0, P, 7):
+        if bsr:
+            eb, es = oracle.bsr_intersect("branchless_merge_bsr", apool[aoff[p]:aoff[p + 1]],
+                                          aspool[aoff[p]:aoff[p + 1]], pool[b_beg[p]:b_end[p]],
+                                          spool[b_beg[p]:b_end[p]])
+            assert np.array_equal(res[0][2][g_off[p]:g_off[p + 1]], eb), p
+            assert np.array_equal(res[0][3][g_off[p]:g_off[p + 1]], es), p
+        else:
+            exp = oracle.branchless_merge(apool[aoff[p]:aoff[p + 1]], pool[b_beg[p]:b_end[p]])
+            assert np.array_equal(res[0][2][g_off[p]:g_off[p + 1]], exp), p
 
 
 # ---- graph edges (count-only gather batches) ------------------------------------------------------
@@ -498,3 +566,18 @@ def test_batch_graph_replays_match():
         torch.cuda.synchronize()
         assert db.totals()[0] == int(counts.sum()), rep
     db.close()
+
+
+def test_datafile_pairs_through_every_entry(tmp_path):
+    """A datafile written in the reference's format is read back and intersected by every entry."""
+    from simd_set_operations_amd import datafile
+
+    rng = np.random.Generator(np.random.PCG64(103))
+    a, b = sorted_set(rng, 5000, 40000), sorted_set(rng, 9000, 40000)
+    p = tmp_path / "0"
+    datafile.write(p, [a, b])
+    sa, sb = datafile.read(p)
+    announce("datafile pair")
+    exp = oracle.branchless_merge(sa, sb)
+    for name in PLAIN:
+        assert np.array_equal(intersect.run_2set(sa, sb, name), exp), name
