@@ -26,17 +26,23 @@ struct MergeSmem {
     // CSR: the A run and the B run (+ alignment shifts); gather: every window rounded out to
     // whole 16-byte chunks (<= 7 extra words per A and B window, +1 boundary element)
     static constexpr int KEYS = GATHER ? U + 16 * MAXSEG + 16 : U + 32;
-    uint32_t key[KEYS];
+    // + K + 8: the merge steps of a lane may read up to K words past its windows
+    uint32_t key[KEYS + K + 8];
     uint32_t st[BSR ? KEYS : 4];
-    uint32_t seg_u[MAXSEG + 1];  // tile-relative unit where segment s starts; [nseg] = tile length
-    uint32_t seg_la[MAXSEG];     // LDS index of the segment's first A element in the window
-    uint32_t seg_lb[MAXSEG];     // LDS index of the segment's first B element in the window
-    uint32_t seg_wa[MAXSEG];     // A elements in the window
-    uint32_t seg_wb[MAXSEG];     // B elements in the window (the boundary element excluded)
-    uint32_t seg_nb[MAXSEG];     // B elements readable from the window start (wb, +1 with a boundary)
-    uint32_t seg_flags[MAXSEG];  // bit0: pair starts in this tile, bit1: pair ends in this tile
-    uint32_t lane_mask[NT];      // per-lane step mask of matches
-    uint32_t lane_pfx[NT];       // per-lane exclusive prefix of matches
+    static_assert(KEYS + K + 8 < 65536 && U < 65536, "segment fields are 16-bit LDS indices");
+    // plain merge: per-segment result prefixes replace the per-lane masks (LDS is what limits how
+    // many workgroups share a CU, so the segment table is 16-bit)
+    static constexpr bool LEAN = !BSR && !BLOCKP;
+    uint16_t seg_u[MAXSEG + 1];  // tile-relative unit where segment s starts; [nseg] = tile length
+    uint16_t seg_la[MAXSEG];     // LDS index of the segment's first A element in the window
+    uint16_t seg_lb[MAXSEG];     // LDS index of the segment's first B element in the window
+    uint16_t seg_wa[MAXSEG];     // A elements in the window
+    uint16_t seg_wb[MAXSEG];     // B elements in the window (the boundary element excluded)
+    uint16_t seg_nb[MAXSEG];     // B elements readable from the window start (wb, +1 with a boundary)
+    uint8_t seg_flags[MAXSEG];   // bit0: pair starts in this tile, bit1: pair ends in this tile
+    uint32_t seg_pre[LEAN ? MAXSEG + 1 : 1];  // plain merge: the tile's results before segment s
+    uint32_t lane_mask[LEAN ? 1 : NT];        // other policies: per-lane step mask of matches
+    uint32_t lane_pfx[LEAN ? 1 : NT];         // other policies: per-lane exclusive prefix of matches
     uint32_t lane_nfirst[BLOCKP ? NT : 1];  // block policies: lane's A elements before a pair boundary
     uint32_t scan_scratch[NT / kWave];
     uint32_t wave_el[NT / kWave];
@@ -45,6 +51,10 @@ struct MergeSmem {
     uint64_t win_g[GATHER ? 2 * MAXSEG : 1];
     uint32_t win_src[GATHER ? 2 * MAXSEG : 1];
 };
+
+// Merge steps re-read both current values (two LDS reads, no selects) rather than the one that
+// changed (one read, three selects).
+constexpr bool kMergeTwoReads = true;
 
 // Lane policies: how a lane intersects its slice of the tile.
 enum : int {
@@ -133,6 +143,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                                                    uint32_t nseg, uint32_t ulen, uint32_t p_first)
 {
     constexpr bool BLOCKP = LP != LP_MERGE;
+    using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const uint32_t wave = tid / kWave;
@@ -147,6 +158,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         uint32_t svals[(BSR && !BLOCKP) ? K : 1];
         uint32_t nelems = 0;
         uint32_t kt = K;  // merge policy: step at which the lane enters the next pair
+        uint32_t s_first = 0;  // plain merge: the lane's first segment
+        bool lane_on = false;  // plain merge: the lane has units in the tile
         // block policies: smask bit x is the lane's x-th A element (first sub-window, then the
         // next pair's); pa0 / pa1 locate them, pb0/nb0 and pb1/nb1 the B sub-windows (BSR output)
         uint32_t nfirst = 0, pb0 = 0, nb0 = 0, pb1 = 0, nb1 = 0;
@@ -191,6 +204,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         } else {
         uint32_t kmax = 0;  // steps to run
         uint32_t pa = 0, pb = 0, ea = 0, eb = 0, ebp = 0, s = 0;
+        lane_on = ub < ulen;
         if (ub < ulen) {
             uint32_t lo = 0, hi = nseg;  // largest s with seg_u[s] <= ub (seg_u[0] == 0)
             while (hi - lo > 1) {
@@ -220,77 +234,139 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             const uint32_t send = sm.seg_u[s + 1];  // > ub
             if (send - ub < kmax) kt = send - ub;   // every pair owns >= c0 >= K units: at most one
         }
-        pa0 = pa;
-        uint32_t a = sm.key[pa], b = sm.key[pb];
-        // One merge step, branch-free: consume A when it is not exhausted and not larger than the
-        // current B (or B is exhausted); emit when the consumed A equals the current B.  One LDS
-        // read per step (the side that advanced).
-        auto step = [&](uint32_t k) {
-            const bool aok = pa < ea, bok = pb < eb;
-            const bool takeA = aok & ((!bok) | (a <= b));
-            bool hit = takeA & (pb < ebp) & (a == b);
-            if constexpr (BSR) {
-                uint32_t stv = 0;
-                if (hit) stv = sm.st[pa] & sm.st[pb];
-                hit = hit && stv != 0;
-                svals[k] = stv;
-                nelems += __popc(stv);
+        if constexpr (!BSR) {
+            // One branch-free loop of K merge steps for every wave.  What a checked merge step
+            // would test each time is settled per lane before and after the loop:
+            //  - a lane whose units run into the next pair (at step kt < K) spends steps [0, kt)
+            //    in its pair's padding (every pair ends with c0 >= K padding units): it starts at
+            //    the next pair's window instead, keeps K - kt steps, and its masks move up by kt;
+            //  - once a side is used up, the steps read past its window and decide arbitrarily,
+            //    so a step emits only while both sides still hold real elements: the A element
+            //    taken exists (fewer than rA A elements taken before it) and so does the current
+            //    B element (fewer than rB taken before it, or the staged element after the
+            //    window); steps past the tile's end emit nothing.
+            uint32_t shift = 0;
+            s_first = s;
+            if (kt < (uint32_t)K) {
+                shift = kt;
+                s++;
+                pa = sm.seg_la[s];
+                pb = sm.seg_lb[s];
+                ea = pa + sm.seg_wa[s];
+                eb = pb + sm.seg_wb[s];
+                ebp = pb + sm.seg_nb[s];
             }
-            smask |= hit ? (1u << k) : 0u;
-            tmask |= takeA ? (1u << k) : 0u;
-            pa += takeA;
-            pb += (!takeA) & bok;
-            const uint32_t v = sm.key[takeA ? pa : pb];
-            a = takeA ? v : a;
-            b = takeA ? b : v;
-        };
-        // Fast path: K merge steps stay inside both windows for every lane of the wave (no early
-        // end, no pair change, >= K elements left on each side), so neither side can run out and
-        // the boundary element cannot be reached: take A iff a <= b, and a == b is a hit.
-        auto step_fast = [&](uint32_t k) {
-            const bool takeA = a <= b;
-            bool hit = a == b;
-            if constexpr (BSR) {
-                uint32_t stv = 0;
-                if (hit) stv = sm.st[pa] & sm.st[pb];
-                hit = hit && stv != 0;
-                svals[k] = stv;
-                nelems += __popc(stv);
-            }
-            smask |= hit ? (1u << k) : 0u;
-            tmask |= takeA ? (1u << k) : 0u;
-            pa += takeA;
-            pb += !takeA;
-            const uint32_t v = sm.key[takeA ? pa : pb];
-            a = takeA ? v : a;
-            b = takeA ? b : v;
-        };
-        const bool uniform = ballot(kmax != (uint32_t)K || kt != (uint32_t)K) == 0;
-        const bool fast = uniform && ballot(pa + K > ea || pb + K > eb) == 0;
-        if (fast) {
-#pragma unroll
-            for (int k = 0; k < K; k++) step_fast((uint32_t)k);
-        } else if (uniform) {
-            // no lane of this wave ends its tile early or crosses into another pair
-#pragma unroll
-            for (int k = 0; k < K; k++) step((uint32_t)k);
-        } else {
+            const uint32_t kv = kmax - shift;
+            pa0 = pa;
+            pa1 = pa;
+            uint32_t a = sm.key[pa], b = sm.key[pb];
+            const uint32_t sum = pa + pb;  // A position + B position - steps taken
+            uint32_t ia = pa;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                if ((uint32_t)k == kt) {
-                    // next pair: its window starts at the pair's first elements
-                    s++;
-                    pa = sm.seg_la[s];
-                    pb = sm.seg_lb[s];
-                    pa1 = pa;
-                    ea = pa + sm.seg_wa[s];
-                    eb = pb + sm.seg_wb[s];
-                    ebp = pb + sm.seg_nb[s];
-                    a = sm.key[pa];
-                    b = sm.key[pb];
+                const bool takeA = a <= b;
+                smask |= (a == b) ? (1u << k) : 0u;
+                tmask |= takeA ? (1u << k) : 0u;
+                ia += takeA ? 1u : 0u;
+                if constexpr (kMergeTwoReads) {
+                    a = sm.key[ia];
+                    b = sm.key[sum + (uint32_t)(k + 1) - ia];
+                } else {
+                    const uint32_t v = sm.key[takeA ? ia : sum + (uint32_t)(k + 1) - ia];
+                    a = takeA ? v : a;
+                    b = takeA ? b : v;
                 }
-                if ((uint32_t)k < kmax) step((uint32_t)k);
             }
+            smask &= (kv >= 32u) ? ~0u : ((1u << kv) - 1u);
+            const uint32_t rA = ea - pa;
+            const uint32_t rB = (ebp > eb) ? ~0u : eb - pb;
+            const bool edge = (rA < (uint32_t)K || rB < (uint32_t)K) && smask != 0u;
+            if (ballot(edge) != 0 && edge) {
+                uint32_t m = smask, keep = 0;
+                while (m) {
+                    const uint32_t k = __builtin_ctz(m);
+                    m &= m - 1;
+                    const uint32_t ta = __popc(tmask & ((1u << k) - 1u));  // A taken before step k
+                    keep |= (ta < rA && k - ta < rB) ? (1u << k) : 0u;
+                }
+                smask = keep;
+            }
+            smask <<= shift;
+            tmask <<= shift;
+        } else {
+            pa0 = pa;
+            uint32_t a = sm.key[pa], b = sm.key[pb];
+            // One merge step, branch-free: consume A when it is not exhausted and not larger than the
+            // current B (or B is exhausted); emit when the consumed A equals the current B.  One LDS
+            // read per step (the side that advanced).
+            auto step = [&](uint32_t k) {
+                const bool aok = pa < ea, bok = pb < eb;
+                const bool takeA = aok & ((!bok) | (a <= b));
+                bool hit = takeA & (pb < ebp) & (a == b);
+                if constexpr (BSR) {
+                    uint32_t stv = 0;
+                    if (hit) stv = sm.st[pa] & sm.st[pb];
+                    hit = hit && stv != 0;
+                    svals[k] = stv;
+                    nelems += __popc(stv);
+                }
+                smask |= hit ? (1u << k) : 0u;
+                tmask |= takeA ? (1u << k) : 0u;
+                pa += takeA;
+                pb += (!takeA) & bok;
+                const uint32_t v = sm.key[takeA ? pa : pb];
+                a = takeA ? v : a;
+                b = takeA ? b : v;
+            };
+            // Fast path: K merge steps stay inside both windows for every lane of the wave (no early
+            // end, no pair change, >= K elements left on each side), so neither side can run out and
+            // the boundary element cannot be reached: take A iff a <= b, and a == b is a hit.
+            auto step_fast = [&](uint32_t k) {
+                const bool takeA = a <= b;
+                bool hit = a == b;
+                if constexpr (BSR) {
+                    uint32_t stv = 0;
+                    if (hit) stv = sm.st[pa] & sm.st[pb];
+                    hit = hit && stv != 0;
+                    svals[k] = stv;
+                    nelems += __popc(stv);
+                }
+                smask |= hit ? (1u << k) : 0u;
+                tmask |= takeA ? (1u << k) : 0u;
+                pa += takeA;
+                pb += !takeA;
+                const uint32_t v = sm.key[takeA ? pa : pb];
+                a = takeA ? v : a;
+                b = takeA ? b : v;
+            };
+            const bool uniform = ballot(kmax != (uint32_t)K || kt != (uint32_t)K) == 0;
+            const bool fast = uniform && ballot(pa + K > ea || pb + K > eb) == 0;
+            if (fast) {
+    #pragma unroll
+                for (int k = 0; k < K; k++) step_fast((uint32_t)k);
+            } else if (uniform) {
+                // no lane of this wave ends its tile early or crosses into another pair
+    #pragma unroll
+                for (int k = 0; k < K; k++) step((uint32_t)k);
+            } else {
+    #pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if ((uint32_t)k == kt) {
+                        // next pair: its window starts at the pair's first elements
+                        s++;
+                        pa = sm.seg_la[s];
+                        pb = sm.seg_lb[s];
+                        pa1 = pa;
+                        ea = pa + sm.seg_wa[s];
+                        eb = pb + sm.seg_wb[s];
+                        ebp = pb + sm.seg_nb[s];
+                        a = sm.key[pa];
+                        b = sm.key[pb];
+                    }
+                    if ((uint32_t)k < kmax) step((uint32_t)k);
+                }
+            }
+
         }
         }
         const uint32_t nwords = __popc(smask);
@@ -299,8 +375,16 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         // ---- 4. tile scan -----------------------------------------------------------------------
         uint32_t tot = 0;
         const uint32_t pfx = block_exclusive_scan<NT>(nwords, sm.scan_scratch, &tot);
-        sm.lane_mask[tid] = smask;
-        sm.lane_pfx[tid] = pfx;
+        if constexpr (Smem::LEAN) {
+            // the segment that starts in this lane's units starts at its first unit, or at step
+            // kt after padding: either way no result of the lane precedes it
+            if (lane_on && sm.seg_u[s_first] == ub) sm.seg_pre[s_first] = pfx;
+            if (lane_on && kt < (uint32_t)K) sm.seg_pre[s_first + 1] = pfx;
+            if (tid == 0) sm.seg_pre[nseg] = tot;
+        } else {
+            sm.lane_mask[tid] = smask;
+            sm.lane_pfx[tid] = pfx;
+        }
         if constexpr (BLOCKP) sm.lane_nfirst[tid] = nfirst;
         if constexpr (BSR) {
             uint32_t el = nelems;
@@ -376,8 +460,14 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 if constexpr (BLOCKP) nb_ = (k == 0) ? 0u : sm.lane_nfirst[l];
                 return sm.lane_pfx[l] + __popc(sm.lane_mask[l] & ((1u << nb_) - 1u));
             };
-            const uint32_t c0v = before(sm.seg_u[tid]);
-            const uint32_t c = before(sm.seg_u[tid + 1]) - c0v;
+            uint32_t c0v, c;
+            if constexpr (Smem::LEAN) {
+                c0v = sm.seg_pre[tid];
+                c = sm.seg_pre[tid + 1] - c0v;
+            } else {
+                c0v = before(sm.seg_u[tid]);
+                c = before(sm.seg_u[tid + 1]) - c0v;
+            }
             const uint32_t f = sm.seg_flags[tid];
             const uint64_t p = (uint64_t)p_first + tid;
             if (f & 1u) plan.pair_local[p] = c0v;
@@ -431,10 +521,10 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             const uint32_t n_a = bad ? 0u : (uint32_t)(ga1 - ga0);
             const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
             const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
-            const uint32_t boff = 4u * ((shift_a + n_a + 3) / 4);
+            constexpr int QRUN = TwoRunStage<NT, 1>::iterations(U + 1);
+            const uint32_t boff = TwoRunStage<NT, QRUN>::b_offset(shift_a, n_a);
             // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks);
             // the staging loads and the segment table's loads are in flight together
-            constexpr int QRUN = (U + 12 + 4 * NT - 1) / (4 * NT);
             TwoRunStage<NT, QRUN> rk, rs;
             if (!bad) {
                 rk.issue(in.A, in.a_pool, ga0, n_a, in.B, in.b_pool, gb0, n_b);

@@ -52,7 +52,7 @@ int fail(int code, const char *fmt, ...)
 // merge policy: kNTM lanes x kK merge-path steps per tile
 constexpr int kNT = 256;  // workgroup size of the search tile
 constexpr int kNTM = 256;
-constexpr int kK = 16;
+constexpr int kK = 20;
 constexpr int kU = kNTM * kK;
 constexpr int kMaxSeg = 130;
 // every pair owns at least c0 units so a tile touches at most kMaxSeg pairs

@@ -504,17 +504,32 @@ __device__ __forceinline__ uint4 load_chunk(const uint32_t *__restrict__ src, ui
 }
 
 // Stages two runs into LDS as whole aligned 16-byte chunks: A[ga0, ga0+nA) at dst[(ga0&3) ...] and
-// B[gb0, gb0+nB) right after the A run's chunks (at dst[4*cA + (gb0&3) ...]).  Chunk c of the
-// combined list goes to dst[4c].  issue() sends every load of the lane (chunks tid, tid+NT, ...)
-// before anything waits on them, store() writes them to LDS: a tile's staging costs one memory
-// round trip, and the caller can issue its other loads in between.  QMAX * NT must cover the
-// chunk count.
+// B[gb0, gb0+nB) right after the A run's chunks (at dst[4*cA + (gb0&3) ...]).  Iteration k of a
+// lane reads chunk tid + k*NT of the A run while k < kA = ceil(cA/NT), then chunk
+// tid + (k-kA)*NT of the B run, so the run (and a row's base address) is the same for every lane
+// of an iteration.  issue() sends every load of the lane before anything waits on them, store()
+// writes them to LDS: a tile's staging costs one memory round trip, and the caller can issue its
+// other loads in between.  QMAX * NT must cover ceil(cA/NT) + ceil(cB/NT) iterations.
 template <int NT, int QMAX>
 struct TwoRunStage {
     uint4 v[QMAX];
     uint32_t tail = 0;  // chunks that end past their pool: loaded element-wise in store()
-    uint32_t cA = 0, total = 0;
+    uint32_t cA = 0, cB = 0, kA = 0;
     uint64_t abase = 0, bbase = 0;
+
+    // iterations needed for runs of at most `units` elements in total (nA + nB <= units)
+    static constexpr int iterations(int units) { return ((units + 6) / 4 + 2 + NT - 1) / NT + 1; }
+    // LDS word offset of the B run's first chunk
+    static __device__ __forceinline__ uint32_t b_offset(uint32_t shift_a, uint32_t nA)
+    {
+        return 4u * ((shift_a + nA + 3) / 4);
+    }
+    // iteration k: first chunk of its row within its run, and whether the run is A (uniform)
+    __device__ __forceinline__ uint32_t row(int k, bool &isA) const
+    {
+        isA = (uint32_t)k < kA;
+        return (isA ? (uint32_t)k : (uint32_t)k - kA) * NT;
+    }
 
     __device__ __forceinline__ void issue(const uint32_t *__restrict__ A, uint64_t apool, uint64_t ga0, uint32_t nA,
                                           const uint32_t *__restrict__ B, uint64_t bpool, uint64_t gb0, uint32_t nB)
@@ -522,21 +537,26 @@ struct TwoRunStage {
         abase = ga0 & ~3ull;
         bbase = gb0 & ~3ull;
         cA = ((uint32_t)(ga0 - abase) + nA + 3) / 4;
-        const uint32_t cB = ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
-        total = cA + cB;
+        cB = ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
+        kA = (cA + NT - 1) / NT;
         // uniform: each run's first chunk and how many of its chunks lie wholly inside the pool
         const uint4 *pa = reinterpret_cast<const uint4 *>(A + abase);
         const uint4 *pb = reinterpret_cast<const uint4 *>(B + bbase);
         const uint64_t fa = (apool - abase) / 4, fb = (bpool - bbase) / 4;
         const uint32_t fullA = fa < cA ? (uint32_t)fa : cA, fullB = fb < cB ? (uint32_t)fb : cB;
+        const bool whole = fullA == cA && fullB == cB;
 #pragma unroll
         for (int k = 0; k < QMAX; k++) {
-            const uint32_t c = threadIdx.x + (uint32_t)k * NT;
-            const bool isA = c < cA;
-            const uint32_t cl = isA ? c : c - cA;
-            const bool full = c < total && cl < (isA ? fullA : fullB);
-            tail |= (c < total && !full) ? (1u << k) : 0u;
-            if (full) v[k] = (isA ? pa : pb)[cl];
+            bool isA;
+            const uint32_t r0 = row(k, isA);
+            const uint4 *base = (isA ? pa : pb) + r0;
+            const uint32_t full = isA ? fullA : fullB, n = isA ? cA : cB;
+            const uint32_t lim = full > r0 ? full - r0 : 0u;  // lanes of this row with a whole chunk
+            if (threadIdx.x < lim) v[k] = base[threadIdx.x];
+            if (!whole) {
+                const uint32_t c = r0 + threadIdx.x;
+                tail |= (c >= full && c < n) ? (1u << k) : 0u;
+            }
         }
     }
 
@@ -547,17 +567,20 @@ struct TwoRunStage {
 #pragma unroll
             for (int k = 0; k < QMAX; k++) {
                 if (tail & (1u << k)) {
-                    const uint32_t c = threadIdx.x + (uint32_t)k * NT;
-                    const bool isA = c < cA;
-                    v[k] = load_chunk(isA ? A : B, isA ? apool : bpool,
-                                      isA ? abase + 4ull * c : bbase + 4ull * (c - cA));
+                    bool isA;
+                    const uint32_t c = row(k, isA) + threadIdx.x;
+                    v[k] = load_chunk(isA ? A : B, isA ? apool : bpool, (isA ? abase : bbase) + 4ull * c);
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < QMAX; k++) {
-            const uint32_t c = threadIdx.x + (uint32_t)k * NT;
-            if (c < total) *reinterpret_cast<uint4 *>(dst + 4 * c) = v[k];
+            bool isA;
+            const uint32_t r0 = row(k, isA);
+            const uint32_t n = isA ? cA : cB;
+            const uint32_t lim = n > r0 ? n - r0 : 0u;
+            if (threadIdx.x < lim)
+                *reinterpret_cast<uint4 *>(dst + 4 * ((isA ? 0u : cA) + r0 + threadIdx.x)) = v[k];
         }
     }
 };
