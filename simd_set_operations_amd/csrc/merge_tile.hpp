@@ -27,8 +27,8 @@ struct MergeSmem {
     // whole 16-byte chunks (<= 7 extra words per A and B window, +1 boundary element)
     static constexpr int KEYS = GATHER ? U + 16 * MAXSEG + 16 : U + 32;
     // + K + 8: the merge steps of a lane may read up to K words past its windows
-    uint32_t key[KEYS + K + 8];
-    uint32_t st[BSR ? KEYS : 4];
+    alignas(16) uint32_t key[KEYS + K + 8];
+    alignas(16) uint32_t st[BSR ? KEYS : 4];
     static_assert(KEYS + K + 8 < 65536 && U < 65536, "segment fields are 16-bit LDS indices");
     // plain merge: per-segment result prefixes replace the per-lane masks (LDS is what limits how
     // many workgroups share a CU, so the segment table is 16-bit)
@@ -521,20 +521,19 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             const uint32_t n_a = bad ? 0u : (uint32_t)(ga1 - ga0);
             const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
             const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
-            constexpr int QRUN = TwoRunStage<NT, 1>::iterations(U + 1);
-            const uint32_t boff = TwoRunStage<NT, QRUN>::b_offset(shift_a, n_a);
-            // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks);
-            // the staging loads and the segment table's loads are in flight together
-            TwoRunStage<NT, QRUN> rk, rs;
+            const uint32_t cA = (shift_a + n_a + 3) / 4, cB = (shift_b + n_b + 3) / 4;
+            const uint32_t boff = 4u * cA;
+            // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks),
+            // loaded straight into LDS; the segment table's loads are in flight meanwhile
             if (!bad) {
-                rk.issue(in.A, in.a_pool, ga0, n_a, in.B, in.b_pool, gb0, n_b);
-                if constexpr (BSR) rs.issue(in.As, in.a_pool, ga0, n_a, in.Bs, in.b_pool, gb0, n_b);
+                stage_run_lds<NT>(in.A, in.a_pool, ga0 & ~3ull, cA, sm.key);
+                stage_run_lds<NT>(in.B, in.b_pool, gb0 & ~3ull, cB, sm.key + boff);
+                if constexpr (BSR) {
+                    stage_run_lds<NT>(in.As, in.a_pool, ga0 & ~3ull, cA, sm.st);
+                    stage_run_lds<NT>(in.Bs, in.b_pool, gb0 & ~3ull, cB, sm.st + boff);
+                }
             }
             if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
-            if (!bad) {
-                rk.store(in.A, in.a_pool, in.B, in.b_pool, sm.key);
-                if constexpr (BSR) rs.store(in.As, in.a_pool, in.Bs, in.b_pool, sm.st);
-            }
             if (tid < nseg) {
                 sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
                 sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);
@@ -592,6 +591,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             sm.seg_nb[tid] = wb + (w.j1 < w.nb ? 1u : 0u);
         }
         if (tid == 0) sm.seg_u[nseg] = ulen;
+        if constexpr (!GATHER) __builtin_amdgcn_s_waitcnt(0);  // the direct-to-LDS loads have landed
         __syncthreads();
 
         merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first);

@@ -503,87 +503,26 @@ __device__ __forceinline__ uint4 load_chunk(const uint32_t *__restrict__ src, ui
     return v;
 }
 
-// Stages two runs into LDS as whole aligned 16-byte chunks: A[ga0, ga0+nA) at dst[(ga0&3) ...] and
-// B[gb0, gb0+nB) right after the A run's chunks (at dst[4*cA + (gb0&3) ...]).  Iteration k of a
-// lane reads chunk tid + k*NT of the A run while k < kA = ceil(cA/NT), then chunk
-// tid + (k-kA)*NT of the B run, so the run (and a row's base address) is the same for every lane
-// of an iteration.  issue() sends every load of the lane before anything waits on them, store()
-// writes them to LDS: a tile's staging costs one memory round trip, and the caller can issue its
-// other loads in between.  QMAX * NT must cover ceil(cA/NT) + ceil(cB/NT) iterations.
-template <int NT, int QMAX>
-struct TwoRunStage {
-    uint4 v[QMAX];
-    uint32_t tail = 0;  // chunks that end past their pool: loaded element-wise in store()
-    uint32_t cA = 0, cB = 0, kA = 0;
-    uint64_t abase = 0, bbase = 0;
-
-    // iterations needed for runs of at most `units` elements in total (nA + nB <= units)
-    static constexpr int iterations(int units) { return ((units + 6) / 4 + 2 + NT - 1) / NT + 1; }
-    // LDS word offset of the B run's first chunk
-    static __device__ __forceinline__ uint32_t b_offset(uint32_t shift_a, uint32_t nA)
-    {
-        return 4u * ((shift_a + nA + 3) / 4);
+// Stages chunks [0, n) of src + base (base a multiple of 4 elements) into dst[4c .. 4c+4) with
+// direct global->LDS loads (16 bytes per lane; a wave's row of 64 chunks lands contiguously at
+// the row's LDS address).  The chunk that reaches past the pool (at most the last one) is written
+// element-wise, zero-filled.  The caller waits (s_waitcnt) and synchronizes before reading dst.
+template <int NT>
+__device__ __forceinline__ void stage_run_lds(const uint32_t *__restrict__ src, uint64_t pool, uint64_t base,
+                                              uint32_t n, uint32_t *dst)
+{
+    const uint64_t fp = pool > base ? (pool - base) / 4 : 0;
+    const uint32_t full = fp < n ? (uint32_t)fp : n;
+    const uint4 *p = reinterpret_cast<const uint4 *>(src + base);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    for (uint32_t r = threadIdx.x - lane; r < full; r += NT) {  // r: the wave's row (uniform)
+        if (r + lane < full) __builtin_amdgcn_global_load_lds(p + r + lane, dst + 4 * r, 16, 0, 0);
     }
-    // iteration k: first chunk of its row within its run, and whether the run is A (uniform)
-    __device__ __forceinline__ uint32_t row(int k, bool &isA) const
-    {
-        isA = (uint32_t)k < kA;
-        return (isA ? (uint32_t)k : (uint32_t)k - kA) * NT;
+    if (full < n && threadIdx.x == 0) {
+        const uint64_t e = base + 4ull * full;
+        for (uint32_t j = 0; j < 4; j++) dst[4 * full + j] = (e + j < pool) ? src[e + j] : 0u;
     }
-
-    __device__ __forceinline__ void issue(const uint32_t *__restrict__ A, uint64_t apool, uint64_t ga0, uint32_t nA,
-                                          const uint32_t *__restrict__ B, uint64_t bpool, uint64_t gb0, uint32_t nB)
-    {
-        abase = ga0 & ~3ull;
-        bbase = gb0 & ~3ull;
-        cA = ((uint32_t)(ga0 - abase) + nA + 3) / 4;
-        cB = ((uint32_t)(gb0 - bbase) + nB + 3) / 4;
-        kA = (cA + NT - 1) / NT;
-        // uniform: each run's first chunk and how many of its chunks lie wholly inside the pool
-        const uint4 *pa = reinterpret_cast<const uint4 *>(A + abase);
-        const uint4 *pb = reinterpret_cast<const uint4 *>(B + bbase);
-        const uint64_t fa = (apool - abase) / 4, fb = (bpool - bbase) / 4;
-        const uint32_t fullA = fa < cA ? (uint32_t)fa : cA, fullB = fb < cB ? (uint32_t)fb : cB;
-        const bool whole = fullA == cA && fullB == cB;
-#pragma unroll
-        for (int k = 0; k < QMAX; k++) {
-            bool isA;
-            const uint32_t r0 = row(k, isA);
-            const uint4 *base = (isA ? pa : pb) + r0;
-            const uint32_t full = isA ? fullA : fullB, n = isA ? cA : cB;
-            const uint32_t lim = full > r0 ? full - r0 : 0u;  // lanes of this row with a whole chunk
-            if (threadIdx.x < lim) v[k] = base[threadIdx.x];
-            if (!whole) {
-                const uint32_t c = r0 + threadIdx.x;
-                tail |= (c >= full && c < n) ? (1u << k) : 0u;
-            }
-        }
-    }
-
-    __device__ __forceinline__ void store(const uint32_t *__restrict__ A, uint64_t apool,
-                                          const uint32_t *__restrict__ B, uint64_t bpool, uint32_t *dst)
-    {
-        if (tail) {
-#pragma unroll
-            for (int k = 0; k < QMAX; k++) {
-                if (tail & (1u << k)) {
-                    bool isA;
-                    const uint32_t c = row(k, isA) + threadIdx.x;
-                    v[k] = load_chunk(isA ? A : B, isA ? apool : bpool, (isA ? abase : bbase) + 4ull * c);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < QMAX; k++) {
-            bool isA;
-            const uint32_t r0 = row(k, isA);
-            const uint32_t n = isA ? cA : cB;
-            const uint32_t lim = n > r0 ? n - r0 : 0u;
-            if (threadIdx.x < lim)
-                *reinterpret_cast<uint4 *>(dst + 4 * ((isA ? 0u : cA) + r0 + threadIdx.x)) = v[k];
-        }
-    }
-};
+}
 
 // Gather staging: nwin windows, window w = src_w[g0_w .. g0_w + n_w), each placed in LDS at
 // dst + 4*cpfx[w] + (g0_w & 3) as whole aligned 16-byte chunks.  cpfx is the exclusive prefix of
