@@ -27,6 +27,7 @@ struct MergeSmem {
     // whole 16-byte chunks (<= 7 extra words per A and B window, +1 boundary element)
     static constexpr int KEYS = GATHER ? U + 16 * MAXSEG + 16 : U + 32;
     // + K + 8: the merge steps of a lane may read up to K words past its windows
+    uint32_t key_guard[4];  // read (never used) by a split search that starts at key[0]
     alignas(16) uint32_t key[KEYS + K + 8];
     alignas(16) uint32_t st[BSR ? KEYS : 4];
     static_assert(KEYS + K + 8 < 65536 && U < 65536, "segment fields are 16-bit LDS indices");
@@ -204,6 +205,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         } else {
         uint32_t kmax = 0;  // steps to run
         uint32_t pa = 0, pb = 0, ea = 0, eb = 0, ebp = 0, s = 0;
+        uint32_t q_lo = 0, q_n = 0, q_a4 = ~3u, q_s4 = ~3u;  // the lane's split search (q_n = 0: none)
+        bool q_on = false;
         lane_on = ub < ulen;
         if (ub < ulen) {
             uint32_t lo = 0, hi = nseg;  // largest s with seg_u[s] <= ub (seg_u[0] == 0)
@@ -222,17 +225,39 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 pa = ea;
                 pb = eb;
             } else {
-                uint32_t xlo = (e > wb) ? e - wb : 0u, xhi = (e < wa) ? e : wa;
-                while (xlo < xhi) {
-                    const uint32_t mid = (xlo + xhi) >> 1;
-                    if (sm.key[la + mid] <= sm.key[lb + e - 1 - mid]) xlo = mid + 1; else xhi = mid;
-                }
-                pa = la + xlo;
-                pb = lb + (e - xlo);
+                pa = la;
+                pb = lb + e;
+                q_lo = (e > wb) ? e - wb : 0u;
+                q_n = ((e < wa) ? e : wa) - q_lo;
+                q_a4 = 4u * (la - 1u);
+                q_s4 = 4u * (la + lb + e - 1u);
+                q_on = true;
             }
             kmax = (ulen - ub < (uint32_t)K) ? ulen - ub : (uint32_t)K;
             const uint32_t send = sm.seg_u[s + 1];  // > ub
             if (send - ub < kmax) kt = send - ub;   // every pair owns >= c0 >= K units: at most one
+        }
+        {
+            // The lane's merge-path split: the largest x in [q_lo, q_lo + q_n] with
+            // A[la + x'] <= B[lb + e - 1 - x'] for every x' < x (ties go to A), by binary lifting
+            // with the wave's common step sequence (steps 2^(L-1) .. 1, L the wave's longest
+            // range); a step that would pass the range end stops at it and changes nothing.
+            // Probe x = y - 1 reads A at byte 4(la + y - 1) and B at byte 4(lb + e - y), whose sum
+            // is the lane constant q_s4 (one guard word below key covers la + y - 1 = -1).
+            const uint32_t L = wave_max_u32(32u - (uint32_t)__clz(q_n));
+            const char *kb = reinterpret_cast<const char *>(sm.key);
+            const uint32_t qhi = q_lo + q_n;
+            uint32_t x = q_lo;
+            for (int bit = (int)L - 1; bit >= 0; bit--) {
+                const uint32_t y = min(x + (1u << bit), qhi);
+                const uint32_t oa = q_a4 + 4u * y;
+                if (*reinterpret_cast<const uint32_t *>(kb + oa) <= *reinterpret_cast<const uint32_t *>(kb + (q_s4 - oa)))
+                    x = y;
+            }
+            if (q_on) {
+                pa += x;  // pa / pb hold the window starts la / lb + e here
+                pb -= x;
+            }
         }
         if constexpr (!BSR) {
             // One branch-free loop of K merge steps for every wave.  What a checked merge step

@@ -47,16 +47,32 @@ __device__ __forceinline__ uint64_t group_partition_point(uint64_t lo, uint64_t 
     return lo + (uint64_t)__popcll((ballot(p) >> gbase) & gmask);
 }
 
-// Wave-wide inclusive scan of a u32 (Hillis-Steele over DPP-capable shuffles).
+// Wave-wide inclusive scan of a u32 with DPP: shifts by 1, 2, 4, 8 inside each row of 16 lanes,
+// then row 0's total into row 1 (and row 2's into row 3) by row_bcast:15, then the first half's
+// total into rows 2 and 3 by row_bcast:31.  A lane with no source adds 0.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v)
 {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, kWave);
-        if (l >= (uint32_t)d) v += t;
-    }
+    constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 1, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 2, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 4, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 8, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast15, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast31, 0xc, 0xf, false);
     return v;
+}
+
+// Wave-wide maximum of a u32 (the same DPP pattern as the scan; lane 63 ends with the maximum).
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+    constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 1, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 2, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 4, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 8, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast15, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast31, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1);
 }
 
 // Block-wide exclusive scan; returns the exclusive prefix and writes the block total to *total.
