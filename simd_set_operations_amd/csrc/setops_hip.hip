@@ -64,6 +64,7 @@ constexpr int kCapB = 3072;
 constexpr uint32_t kC0S = (kUS + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
 static_assert(kC0 >= (uint32_t)kK && kC0S >= (uint32_t)kKS, "a lane crosses at most one pair boundary");
 
+constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this many groups run as one kernel
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
 
 enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4 };
@@ -418,14 +419,21 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     }
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev->second, stream));
-    // scan: group sums, one-block scan of the group sums (writes the totals), per-group scan
+    // scan: with few groups one kernel (a workgroup adds the earlier groups' tile counts itself);
+    // otherwise group sums, one-block scan of the group sums (writes the totals), per-group scan
     const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
-    hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
-                       h->plan.gsum2, (uint64_t)ngroups, 1, out.totals ? out.totals : h->d_tot);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, out);
+    const int direct = ngroups <= kScanDirectGroups ? 1 : 0;
+    TileOut so = out;
+    if (!so.totals) so.totals = h->d_tot;
+    if (!direct) {
+        hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                           h->plan.gsum2, (uint64_t)ngroups, 1, so.totals);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, so,
+                       direct);
     HIP_TRY(hipGetLastError());
     // write
     if (!count_only || out.out_off) {

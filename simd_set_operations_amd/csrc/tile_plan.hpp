@@ -334,13 +334,68 @@ __global__ __launch_bounds__(NT) void reduce_tiles_kernel(TilePlan plan)
 // Per-group block scan of the tile counts plus the (already scanned) group base.  The last
 // group writes tile_off[ntiles] and flags a clipped output.  Totals were written by the group
 // scan.
+// With `direct` (few groups) there were no group-sum kernels: a workgroup adds the tile counts
+// of the earlier groups itself (both counts for the last group, which writes the totals).
 template <int NT>
-__global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut out)
+__global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut out, int direct)
 {
     static_assert(NT == (int)kScanGroup, "one thread per tile of a group");
     __shared__ uint32_t scratch[NT / kWave];
+    __shared__ unsigned long long red[NT / kWave][2];
     const uint32_t g = blockIdx.x;
-    const unsigned long long base = plan.gsum[g];
+    unsigned long long base;
+    if (direct) {
+        const uint32_t ngroups = (plan.ntiles + kScanGroup - 1) / kScanGroup;
+        const bool last = g + 1 == ngroups;
+        const uint32_t end = last ? plan.ntiles : g * kScanGroup;  // the last group sums all tiles
+        // four words per load, four loads in flight per thread
+        unsigned long long b = 0, e = 0;
+        for (uint32_t i0 = 4 * threadIdx.x; i0 < end; i0 += 16 * NT) {
+            uint4 c[4], el[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + 4 * NT * k;
+                if (i < end) {
+                    c[k] = *reinterpret_cast<const uint4 *>(plan.tile_count + i);
+                    if (last) el[k] = *reinterpret_cast<const uint4 *>(plan.tile_elems + i);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + 4 * NT * k;
+                if (i < end) {
+                    const uint32_t cw[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (i + q < g * kScanGroup) b += cw[q];
+                        if (last && i + q < end) {
+                            const uint32_t ew[4] = {el[k].x, el[k].y, el[k].z, el[k].w};
+                            e += ew[q];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            b += __shfl_xor(b, d, kWave);
+            e += __shfl_xor(e, d, kWave);
+        }
+        if (lane_id() == 0) {
+            red[threadIdx.x / kWave][0] = b;
+            red[threadIdx.x / kWave][1] = e;
+        }
+        __syncthreads();
+        base = 0;
+        unsigned long long el = 0;
+        for (int k = 0; k < NT / kWave; k++) {
+            base += red[k][0];
+            el += red[k][1];
+        }
+        if (last && threadIdx.x == 0 && out.totals) out.totals[1] = el;  // totals[0] below
+    } else {
+        base = plan.gsum[g];
+    }
     const uint32_t i = g * kScanGroup + threadIdx.x;
     const uint32_t c = (i < plan.ntiles) ? plan.tile_count[i] : 0u;
     uint32_t tot;
@@ -349,14 +404,17 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
     if (i + 1 == plan.ntiles) {
         const unsigned long long total = base + ex + c;
         plan.tile_off[plan.ntiles] = total;
+        if (direct && out.totals) out.totals[0] = total;
         if (out.out && total > out.cap && out.error) atomicOr(out.error, 1u);
     }
 }
 
 // Write: moves each tile's staged results to their final position and turns in-tile pair offsets
-// into output offsets.  A group of 16 lanes per tile (4 tiles per wave), grid-stride.  Staging
-// slots are 16-byte aligned: a lane reads four results with one load (up to QP loads, all issued
-// before any store) and writes them to consecutive output words (any alignment).
+// into output offsets.  A group of 16 lanes per tile (4 tiles per wave), grid-stride.  The memory
+// round trips are few: the tile's descriptor words together, then its staged results (up to QP
+// 16-byte loads per lane) together with the first pairs' offsets, then the stores.  Staging slots
+// are 16-byte aligned; a full group of four results goes out as one 16-byte store (global memory
+// takes them at 4-byte alignment), the tail word by word.
 template <int NT, bool BSR, int QP>
 __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out)
 {
@@ -366,52 +424,73 @@ __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan 
     for (uint64_t t = ((uint64_t)blockIdx.x * NT + threadIdx.x) / G; t < plan.ntiles; t += gstride) {
         const uint64_t base = plan.tile_off[t];
         const uint32_t n = plan.tile_count[t];
-        if (out.out) {
-            const uint32_t *sk = plan.stage + t * plan.stage_words;
-            const uint32_t *ss = sk + plan.stage_words / 2;
-            uint64_t lim = n;
-            if (base + n > out.cap) lim = (base < out.cap) ? out.cap - base : 0;
-            for (uint32_t x0 = 0; x0 < lim; x0 += 4 * G * QP) {
-                uint4 v[QP], w[BSR ? QP : 1];
+        const uint32_t pf = plan.tile_first[t], pl = plan.tile_last[t];
+        const uint32_t *sk = plan.stage + t * plan.stage_words;
+        const uint32_t *ss = sk + plan.stage_words / 2;
+        uint64_t lim = out.out ? n : 0;
+        if (base + lim > out.cap) lim = (base < out.cap) ? out.cap - base : 0;
+        const uint64_t u0 = t * plan.U;
+        // pairs pf+1 .. pl start in this tile, pf does if its first unit does; the in-tile
+        // offsets of the tile's first 4*G pairs load with the first results
+        constexpr int QL = 4;
+        const bool first_here = out.out_off && plan.uoff[pf] >= u0;
+        uint32_t loc[QL];
 #pragma unroll
-                for (int k = 0; k < QP; k++) {
-                    const uint32_t x = x0 + 4 * (g + G * k);
-                    if (x < lim) {
-                        v[k] = *reinterpret_cast<const uint4 *>(sk + x);
-                        if constexpr (BSR) w[k] = *reinterpret_cast<const uint4 *>(ss + x);
-                    }
+        for (int k = 0; k < QL; k++) {
+            const uint32_t p = pf + g + G * k;
+            if (out.out_off && p <= pl) loc[k] = plan.pair_local[p];
+        }
+        for (uint32_t x0 = 0; x0 < lim; x0 += 4 * G * QP) {
+            uint4 v[QP], w[BSR ? QP : 1];
+#pragma unroll
+            for (int k = 0; k < QP; k++) {
+                const uint32_t x = x0 + 4 * (g + G * k);
+                if (x < lim) {
+                    v[k] = *reinterpret_cast<const uint4 *>(sk + x);
+                    if constexpr (BSR) w[k] = *reinterpret_cast<const uint4 *>(ss + x);
                 }
+            }
 #pragma unroll
-                for (int k = 0; k < QP; k++) {
-                    const uint32_t x = x0 + 4 * (g + G * k);
-                    if (x < lim) {
-                        // global memory takes 16-byte stores at 4-byte alignment: one store for a
-                        // full group of four, single words for the tail
-                        const uint64_t m = lim - x;
-                        uint32_t *o = out.out + base + x;
-                        uint32_t *os = BSR ? out.out_s + base + x : nullptr;
-                        if (m >= 4) {
-                            *reinterpret_cast<uint4 *>(o) = v[k];
-                            if constexpr (BSR) *reinterpret_cast<uint4 *>(os) = w[k];
-                        } else {
-                            o[0] = v[k].x;
-                            if (m > 1) o[1] = v[k].y;
-                            if (m > 2) o[2] = v[k].z;
-                            if constexpr (BSR) {
-                                os[0] = w[k].x;
-                                if (m > 1) os[1] = w[k].y;
-                                if (m > 2) os[2] = w[k].z;
-                            }
+            for (int k = 0; k < QP; k++) {
+                const uint32_t x = x0 + 4 * (g + G * k);
+                if (x < lim) {
+                    const uint64_t m = lim - x;
+                    uint32_t *o = out.out + base + x;
+                    uint32_t *os = BSR ? out.out_s + base + x : nullptr;
+                    if (m >= 4) {
+                        *reinterpret_cast<uint4 *>(o) = v[k];
+                        if constexpr (BSR) *reinterpret_cast<uint4 *>(os) = w[k];
+                    } else {
+                        o[0] = v[k].x;
+                        if (m > 1) o[1] = v[k].y;
+                        if (m > 2) o[2] = v[k].z;
+                        if constexpr (BSR) {
+                            os[0] = w[k].x;
+                            if (m > 1) os[1] = w[k].y;
+                            if (m > 2) os[2] = w[k].z;
                         }
                     }
                 }
             }
         }
         if (out.out_off) {
-            const uint64_t u0 = t * plan.U;
-            const uint32_t pf = plan.tile_first[t], pl = plan.tile_last[t];
-            for (uint32_t p = pf + g; p <= pl; p += G)
-                if (plan.uoff[p] >= u0) out.out_off[p] = base + plan.pair_local[p];
+#pragma unroll
+            for (int k = 0; k < QL; k++) {
+                const uint32_t p = pf + g + G * k;
+                if (p <= pl && (p != pf || first_here)) out.out_off[p] = base + loc[k];
+            }
+            for (uint32_t q0 = pf + g + G * QL; q0 <= pl; q0 += G * QL) {  // more than 4*G pairs
+#pragma unroll
+                for (int k = 0; k < QL; k++) {
+                    const uint32_t p = q0 + G * k;
+                    if (p <= pl) loc[k] = plan.pair_local[p];
+                }
+#pragma unroll
+                for (int k = 0; k < QL; k++) {
+                    const uint32_t p = q0 + G * k;
+                    if (p <= pl) out.out_off[p] = base + loc[k];
+                }
+            }
             if (t == plan.ntiles - 1 && g == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
         }
     }
