@@ -53,10 +53,6 @@ struct MergeSmem {
     uint32_t win_src[GATHER ? 2 * MAXSEG : 1];
 };
 
-// Merge steps re-read both current values (two LDS reads, no selects) rather than the one that
-// changed (one read, three selects).
-constexpr bool kMergeTwoReads = true;
-
 // Lane policies: how a lane intersects its slice of the tile.
 enum : int {
     LP_MERGE = 0,    // K branch-free merge steps (merge.rs:39-87)
@@ -260,7 +256,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             }
         }
         if constexpr (!BSR) {
-            // One branch-free loop of K merge steps for every wave.  What a checked merge step
+            // One branch-free loop of K merge steps for every wave; a step re-reads both current
+            // values (two LDS reads, no selects).  What a checked merge step
             // would test each time is settled per lane before and after the loop:
             //  - a lane whose units run into the next pair (at step kt < K) spends steps [0, kt)
             //    in its pair's padding (every pair ends with c0 >= K padding units): it starts at
@@ -285,22 +282,21 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             pa0 = pa;
             pa1 = pa;
             uint32_t a = sm.key[pa], b = sm.key[pb];
-            const uint32_t sum = pa + pb;  // A position + B position - steps taken
-            uint32_t ia = pa;
+            // oa: byte offset of A's current element; B's is s4 + 4*steps - oa.  The empty asm
+            // keeps oa the one position variable (otherwise the compiler rebuilds both offsets
+            // from separate counters every step).
+            const char *kb = reinterpret_cast<const char *>(sm.key);
+            const uint32_t s4 = 4u * (pa + pb);
+            uint32_t oa = 4u * pa;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const bool takeA = a <= b;
                 smask |= (a == b) ? (1u << k) : 0u;
                 tmask |= takeA ? (1u << k) : 0u;
-                ia += takeA ? 1u : 0u;
-                if constexpr (kMergeTwoReads) {
-                    a = sm.key[ia];
-                    b = sm.key[sum + (uint32_t)(k + 1) - ia];
-                } else {
-                    const uint32_t v = sm.key[takeA ? ia : sum + (uint32_t)(k + 1) - ia];
-                    a = takeA ? v : a;
-                    b = takeA ? b : v;
-                }
+                oa += takeA ? 4u : 0u;
+                asm volatile("" : "+v"(oa));
+                a = *reinterpret_cast<const uint32_t *>(kb + oa);
+                b = *reinterpret_cast<const uint32_t *>(kb + (s4 - oa) + 4u * (uint32_t)(k + 1));
             }
             smask &= (kv >= 32u) ? ~0u : ((1u << kv) - 1u);
             const uint32_t rA = ea - pa;
