@@ -66,6 +66,7 @@ static_assert(kC0 >= (uint32_t)kK && kC0S >= (uint32_t)kKS, "a lane crosses at m
 
 constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this many groups run as one kernel
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
+constexpr int kPlaceQ = 8;  // words per lane per round of the placement kernel
 
 enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4 };
 
@@ -435,14 +436,15 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, so,
                        direct);
     HIP_TRY(hipGetLastError());
-    // write
+    // write: one workgroup per group of kPlaceTPW tiles
     if (!count_only || out.out_off) {
-        const uint32_t gp = grid_for(h->plan.ntiles, 16);  // 16 tiles per workgroup (16 lanes each)
+        const uint32_t gp = grid_for((h->plan.ntiles + kPlaceTPW - 1) / kPlaceTPW, 1);
         if (h->bsr)
-            hipLaunchKernelGGL((place_tiles_kernel<256, true, 4>), dim3(gp), dim3(256), 0, stream, h->in, h->plan, out);
-        else
-            hipLaunchKernelGGL((place_tiles_kernel<256, false, 4>), dim3(gp), dim3(256), 0, stream, h->in, h->plan,
+            hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan,
                                out);
+        else
+            hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, false>), dim3(gp), dim3(256), 0, stream, h->in,
+                               h->plan, out);
         HIP_TRY(hipGetLastError());
     }
     return SETOPS_OK;

@@ -17,7 +17,7 @@
 //   plan     unit prefix (closed form for CSR, a three-kernel scan for gather), tile -> pair maps
 //   split    where each tile starts inside its first pair (merge-path split or probe index)
 //   tile     one workgroup per tile: count and stage the tile's results (merge_tile / search_tile)
-//   scan     group sums -> one-block scan of the group sums -> per-group scan of tile counts
+//   scan     scan of the tile counts (one kernel when the tiles are few)
 //   place    move staged results to their final positions, per-pair output offsets
 #pragma once
 
@@ -26,6 +26,7 @@
 namespace setops {
 
 constexpr uint32_t kScanGroup = 1024;  // elements per group in the three-kernel scans
+constexpr uint32_t kPlaceTPW = 16;     // tiles per workgroup of the placement kernel
 
 struct PairBatch {
     const uint32_t *A;   // A keys (values, or BSR bases)
@@ -409,91 +410,92 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
     }
 }
 
-// Write: moves each tile's staged results to their final position and turns in-tile pair offsets
-// into output offsets.  A group of 16 lanes per tile (4 tiles per wave), grid-stride.  The memory
-// round trips are few: the tile's descriptor words together, then its staged results (up to QP
-// 16-byte loads per lane) together with the first pairs' offsets, then the stores.  Staging slots
-// are 16-byte aligned; a full group of four results goes out as one 16-byte store (global memory
-// takes them at 4-byte alignment), the tail word by word.
-template <int NT, bool BSR, int QP>
+// Write: moves the staged results to their final positions and writes the per-pair output
+// offsets.  One workgroup per group of kPlaceTPW consecutive tiles (grid-stride): their results
+// are one contiguous run of the output, [tile_off[t0], tile_off[t0 + TPW]).  The group's tile
+// offsets (relative to the run's start) go to LDS together with the pair range; then every lane
+// copies words x = lane + NT*j of the run -- it finds the word's tile among the TPW offsets and
+// reads the word from that tile's staging slot -- with all of its loads in flight before its
+// stores, and writes the offsets of the pairs that start in the group (the tile holding a pair's
+// first unit recorded the pair's in-tile result offset).
+template <int NT, int Q, bool BSR>
 __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out)
 {
-    constexpr uint32_t G = 16;
-    const uint32_t g = lane_id() % G;
-    const uint64_t gstride = (uint64_t)gridDim.x * NT / G;
-    for (uint64_t t = ((uint64_t)blockIdx.x * NT + threadIdx.x) / G; t < plan.ntiles; t += gstride) {
-        const uint64_t base = plan.tile_off[t];
-        const uint32_t n = plan.tile_count[t];
-        const uint32_t pf = plan.tile_first[t], pl = plan.tile_last[t];
-        const uint32_t *sk = plan.stage + t * plan.stage_words;
-        const uint32_t *ss = sk + plan.stage_words / 2;
-        uint64_t lim = out.out ? n : 0;
-        if (base + lim > out.cap) lim = (base < out.cap) ? out.cap - base : 0;
-        const uint64_t u0 = t * plan.U;
-        // pairs pf+1 .. pl start in this tile, pf does if its first unit does; the in-tile
-        // offsets of the tile's first 4*G pairs load with the first results
-        constexpr int QL = 4;
-        const bool first_here = out.out_off && plan.uoff[pf] >= u0;
-        uint32_t loc[QL];
-#pragma unroll
-        for (int k = 0; k < QL; k++) {
-            const uint32_t p = pf + g + G * k;
-            if (out.out_off && p <= pl) loc[k] = plan.pair_local[p];
+    constexpr uint32_t TPW = kPlaceTPW;
+    static_assert((TPW & (TPW - 1)) == 0 && TPW < (uint32_t)NT, "a log2(TPW)-step search, one thread per tile offset");
+    __shared__ uint32_t toff[TPW + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t ngrp = (plan.ntiles + TPW - 1) / TPW;
+    const uint32_t sw = plan.stage_words;
+    for (uint64_t w = blockIdx.x; w < ngrp; w += gridDim.x) {
+        const uint64_t tw0 = w * TPW;
+        const uint32_t ntw = (plan.ntiles - tw0 < TPW) ? (uint32_t)(plan.ntiles - tw0) : TPW;
+        const uint64_t base = plan.tile_off[tw0];
+        const uint64_t pf = plan.tile_first[tw0], pl = plan.tile_last[tw0 + ntw - 1];
+        // toff[k]: tile k's first word, relative to the group's (< TPW * stage slot); past the
+        // group's end: all ones, so the search never stops there
+        if (tid <= TPW) toff[tid] = (tid <= ntw) ? (uint32_t)(plan.tile_off[tw0 + tid] - base) : ~0u;
+        // the first pairs' prefix entries and in-tile offsets load with the tile offsets
+        const uint64_t p = pf + tid;
+        const bool pv = out.out_off && p <= pl;
+        uint64_t pu = 0;
+        uint32_t ploc = 0;
+        if (pv) {
+            pu = plan.uoff[p];
+            ploc = plan.pair_local[p];
         }
-        for (uint32_t x0 = 0; x0 < lim; x0 += 4 * G * QP) {
-            uint4 v[QP], w[BSR ? QP : 1];
+        __syncthreads();
+        const uint32_t nw = toff[ntw];
+        if (out.out) {
+            const uint64_t room = out.cap > base ? out.cap - base : 0;
+            const uint32_t lim = (uint64_t)nw < room ? nw : (uint32_t)room;
+            // 32-bit byte offsets from uniform bases (one address register per word)
+            const char *stage = reinterpret_cast<const char *>(plan.stage + tw0 * sw);
+            char *dk = reinterpret_cast<char *>(out.out + base);
+            char *ds = BSR ? reinterpret_cast<char *>(out.out_s + base) : nullptr;
+            const uint32_t sbytes = 2u * sw;  // byte distance from a key to its state (sw/2 words)
+            for (uint32_t x0 = 0; x0 < lim; x0 += (uint32_t)NT * Q) {
+                uint32_t boff[Q];  // staging byte offset (from the group's first slot) of word x
 #pragma unroll
-            for (int k = 0; k < QP; k++) {
-                const uint32_t x = x0 + 4 * (g + G * k);
-                if (x < lim) {
-                    v[k] = *reinterpret_cast<const uint4 *>(sk + x);
-                    if constexpr (BSR) w[k] = *reinterpret_cast<const uint4 *>(ss + x);
+                for (int j = 0; j < Q; j++) {
+                    const uint32_t x = x0 + tid + (uint32_t)j * NT;
+                    // the largest lo < TPW with toff[lo] <= x, in log2(TPW) fixed steps (the words'
+                    // searches are independent, so their LDS reads overlap)
+                    uint32_t lo = 0;
+#pragma unroll
+                    for (uint32_t st = TPW / 2; st >= 1; st >>= 1) lo += (toff[lo + st] <= x) ? st : 0u;
+                    boff[j] = 4u * (lo * sw + (x - toff[lo]));
                 }
-            }
+                uint32_t v[Q], s2[BSR ? Q : 1];
 #pragma unroll
-            for (int k = 0; k < QP; k++) {
-                const uint32_t x = x0 + 4 * (g + G * k);
-                if (x < lim) {
-                    const uint64_t m = lim - x;
-                    uint32_t *o = out.out + base + x;
-                    uint32_t *os = BSR ? out.out_s + base + x : nullptr;
-                    if (m >= 4) {
-                        *reinterpret_cast<uint4 *>(o) = v[k];
-                        if constexpr (BSR) *reinterpret_cast<uint4 *>(os) = w[k];
-                    } else {
-                        o[0] = v[k].x;
-                        if (m > 1) o[1] = v[k].y;
-                        if (m > 2) o[2] = v[k].z;
-                        if constexpr (BSR) {
-                            os[0] = w[k].x;
-                            if (m > 1) os[1] = w[k].y;
-                            if (m > 2) os[2] = w[k].z;
-                        }
+                for (int j = 0; j < Q; j++) {
+                    if (x0 + tid + (uint32_t)j * NT < lim) {
+                        v[j] = *reinterpret_cast<const uint32_t *>(stage + boff[j]);
+                        if constexpr (BSR) s2[j] = *reinterpret_cast<const uint32_t *>(stage + (boff[j] + sbytes));
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < Q; j++) {
+                    const uint32_t x = x0 + tid + (uint32_t)j * NT;
+                    if (x < lim) {
+                        *reinterpret_cast<uint32_t *>(dk + 4u * x) = v[j];
+                        if constexpr (BSR) *reinterpret_cast<uint32_t *>(ds + 4u * x) = s2[j];
                     }
                 }
             }
         }
         if (out.out_off) {
-#pragma unroll
-            for (int k = 0; k < QL; k++) {
-                const uint32_t p = pf + g + G * k;
-                if (p <= pl && (p != pf || first_here)) out.out_off[p] = base + loc[k];
+            const uint64_t U = plan.U, u0 = tw0 * U;
+            // a pair whose first unit lies before the group started in an earlier group
+            if (pv && pu >= u0) out.out_off[p] = base + toff[(uint32_t)(pu - u0) / (uint32_t)U] + ploc;
+            for (uint64_t q = p + NT; q <= pl; q += NT) {  // groups of more than NT pairs
+                const uint64_t uq = plan.uoff[q];
+                if (uq >= u0) out.out_off[q] = base + toff[(uint32_t)(uq - u0) / (uint32_t)U] + plan.pair_local[q];
             }
-            for (uint32_t q0 = pf + g + G * QL; q0 <= pl; q0 += G * QL) {  // more than 4*G pairs
-#pragma unroll
-                for (int k = 0; k < QL; k++) {
-                    const uint32_t p = q0 + G * k;
-                    if (p <= pl) loc[k] = plan.pair_local[p];
-                }
-#pragma unroll
-                for (int k = 0; k < QL; k++) {
-                    const uint32_t p = q0 + G * k;
-                    if (p <= pl) out.out_off[p] = base + loc[k];
-                }
-            }
-            if (t == plan.ntiles - 1 && g == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
         }
+        __syncthreads();  // toff is rewritten by the next group
     }
+    if (out.out_off && blockIdx.x == 0 && tid == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
 }
 
 // ---- per-tile segment geometry -------------------------------------------------------------------
