@@ -451,6 +451,14 @@ def main():
     log(f"step {step_ms:.4f} ms, main kernel {kern_ms:.4f} ms, {value:.1f} Gelem/s, {achieved:.0f} GB/s")
 
     extra = {"count_only": head["count"], "compare": {}}
+    try:
+        from simd_set_operations_amd import tuning
+
+        pred = tuning.predict(np.diff(a_off), np.diff(b_off))
+        extra["autotune"] = {"uneven_batch_choice": min(pred, key=pred.get), "predicted_ms": pred,
+                             "skewed_choice": tuning.choose(np.full(4096, 256), np.full(4096, 1 << 24))}
+    except FileNotFoundError:
+        extra["autotune"] = None
     for name in [x for x in args.compare.split(",") if x and x != args.algo]:
         c = time_batch(name)
         c["roofline_frac"] = Bytes / (c["out"]["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS

@@ -104,6 +104,10 @@ DATAGEN_SIGNATURES = {
         _c.c_int64,
         [_c.c_uint64, _c.c_uint32, _c.c_uint32, _c.c_double, _c.c_double, _c.c_double, _vp, _vp],
     ),
+    "setops_gen_fixed_batch": (
+        _c.c_int,
+        [_c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _c.c_uint64, _vp, _vp],
+    ),
     "setops_gen_uneven_fill": (
         _c.c_int,
         [_c.c_uint64, _c.c_uint64, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _vp, _vp, _vp, _vp, _vp],

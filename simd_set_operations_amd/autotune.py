@@ -1,0 +1,114 @@
+"""Autotuner: batched throughput of every registered 2-set entry over a grid of pair shapes.
+
+Grid: larger-set size 2^6 .. 2^14, skew (larger / smaller) 2^0 .. 2^10, selectivity 0.1 / 0.5 /
+0.9, density 0.1 (element space = 10 x larger), the planted-shared-then-disjoint data of the
+benchmark workloads.  Each cell is a fixed-shape CSR batch of about 2^26 input elements (at most
+2^18 pairs); every entry runs it as a device batch (plan, tile kernel, scan, placement, with
+outputs) and is timed over replayed runs with HIP events, after a check of its per-pair counts
+against the planted counts.  The table (``tuning.json``) feeds :mod:`.tuning`, which prices a
+batch per entry and picks the cheapest (``hip_auto``).
+
+Run on the GPU:  python -m simd_set_operations_amd.autotune --out simd_set_operations_amd/tuning.json
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+ALGOS = ("hip_branchless_merge", "hip_shuffling", "hip_qfilter", "hip_galloping", "hip_binary_search")
+GRID = {
+    "log2_larger": [6, 8, 10, 12, 14],
+    "log2_skew": [0, 3, 6, 10],
+    "selectivity": [0.1, 0.5, 0.9],
+}
+CELL_ELEMENTS = 1 << 26
+MAX_PAIRS = 1 << 18
+
+
+def cell_shape(log2_larger, log2_skew):
+    larger = 1 << log2_larger
+    smaller = max(1, larger >> log2_skew)
+    pairs = int(min(MAX_PAIRS, max(64, CELL_ELEMENTS // (larger + smaller))))
+    return larger, smaller, pairs
+
+
+def time_entry(torch, algo, A, a_off, B, b_off, shared, runs):
+    from .batch import DeviceBatch
+
+    db = DeviceBatch(algo, A, a_off, B, b_off)
+    stream = torch.cuda.current_stream()
+    try:
+        for _ in range(2):
+            db.run(stream)
+        torch.cuda.synchronize()
+        _, cnt, _ = db.results()
+        if not np.all(cnt == shared):
+            raise RuntimeError(f"{algo}: per-pair counts differ from the planted {shared}")
+        for _ in range(2):  # the replayed launch graph
+            db.run(stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(runs):
+            db.run(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / runs
+    finally:
+        db.close()
+
+
+def run(out_path, runs=5, grid=None, algos=ALGOS, log=print):
+    import torch
+
+    from . import datagen, intersect
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the autotuner measures on a GPU; none is visible")
+    intersect.init(0)
+    grid = grid or GRID
+    cells = []
+    t0 = time.perf_counter()
+    for l in grid["log2_larger"]:
+        for s in grid["log2_skew"]:
+            larger, smaller, pairs = cell_shape(l, s)
+            for sel in grid["selectivity"]:
+                A, a_off, B, b_off, shared = datagen.fixed_batch(pairs, larger, smaller, sel)
+                E = int(A.size + B.size)
+                g = {}
+                for algo in algos:
+                    ms = time_entry(torch, algo, A, a_off, B, b_off, shared, runs)
+                    g[algo] = E / (ms / 1e3) / 1e9
+                best = max(g, key=g.get)
+                cells.append({"log2_larger": l, "log2_skew": s, "selectivity": sel, "larger": larger,
+                              "smaller": smaller, "pairs": pairs, "elements": E, "gelems": g, "best": best})
+                log(f"[autotune] L=2^{l} skew=2^{s} sel={sel} pairs={pairs}: best {best} "
+                    + " ".join(f"{a[4:]}={v:.0f}" for a, v in g.items()))
+    table = {
+        "device": torch.cuda.get_device_name(0),
+        "created_unix": int(time.time()),
+        "seconds": round(time.perf_counter() - t0, 1),
+        "unit": "Gelem/s (input elements, sum |A|+|B|, per second; batch with outputs)",
+        "density": 0.1,
+        "cell_elements": CELL_ELEMENTS,
+        "grid": grid,
+        "algorithms": list(algos),
+        "cells": cells,
+    }
+    with open(out_path, "w") as f:
+        json.dump(table, f, indent=1)
+    return table
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--out", default="tuning.json")
+    ap.add_argument("--runs", type=int, default=5)
+    args = ap.parse_args()
+    run(args.out, args.runs, log=lambda m: (print(m), sys.stdout.flush()))
+
+
+if __name__ == "__main__":
+    main()

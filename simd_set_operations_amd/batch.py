@@ -44,6 +44,23 @@ def to_device_u64(arr, torch):
     return torch.from_numpy(a.view(np.int64)).to("cuda")
 
 
+def resolve_algo(algo, a_sizes, b_sizes, bsr):
+    """``hip_auto``: the entry the tuning table prices cheapest for these pair sizes (its BSR twin
+    for BSR sets); any other name is returned as it is."""
+    if algo != "hip_auto":
+        return algo
+    from . import tuning
+
+    best = tuning.choose(a_sizes, b_sizes)
+    if bsr:
+        from .intersect import registered
+
+        twin = best + "_bsr"
+        return twin if twin in registered() else "hip_galloping_bsr" if best == "hip_binary_search" else \
+            "hip_branchless_merge_bsr"
+    return best
+
+
 class _Batch:
     def _outputs(self, capacity):
         torch = self.torch
@@ -81,6 +98,8 @@ class DeviceBatch(_Batch):
             raise ValueError("pool lengths must equal the last offsets")
         self.a_sizes = np.diff(a_off)
         self.b_sizes = np.diff(b_off)
+        self.requested = algo
+        algo = self.algo = resolve_algo(algo, self.a_sizes, self.b_sizes, A_states is not None)
         self.dA = to_device_u32(A, torch)
         self.dB = to_device_u32(B, torch)
         self.dA_off = to_device_u64(a_off, torch)
@@ -191,15 +210,22 @@ class GatherBatch(DeviceBatch):
         self.P = int(len(a_beg))
         if not (len(a_end) == len(b_beg) == len(b_end) == self.P) or self.P == 0:
             raise ValueError("a_beg, a_end, b_beg, b_end must have the same non-zero length")
+        la = lb = None
+        if not _is_tensor(a_beg):
+            la = np.asarray(a_end, np.int64) - np.asarray(a_beg, np.int64)
+            lb = np.asarray(b_end, np.int64) - np.asarray(b_beg, np.int64)
+            if np.any(la < 0) or np.any(lb < 0):
+                raise ValueError("end must not precede begin")
         if capacity is None:
-            if _is_tensor(a_beg):
+            if la is None:
                 capacity = int(torch.minimum(a_end - a_beg, b_end - b_beg).sum().item())
             else:
-                la = np.asarray(a_end, np.int64) - np.asarray(a_beg, np.int64)
-                lb = np.asarray(b_end, np.int64) - np.asarray(b_beg, np.int64)
-                if np.any(la < 0) or np.any(lb < 0):
-                    raise ValueError("end must not precede begin")
                 capacity = int(np.minimum(la, lb).sum())
+        self.requested = algo
+        if algo == "hip_auto" and la is None:
+            la = (a_end - a_beg).cpu().numpy()
+            lb = (b_end - b_beg).cpu().numpy()
+        algo = self.algo = resolve_algo(algo, la, lb, A_states is not None)
         self.dA = to_device_u32(A, torch)
         self.dB = to_device_u32(B, torch)
         self.a_len = int(self.dA.numel()) if len(A) else 0

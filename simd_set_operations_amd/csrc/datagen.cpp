@@ -164,6 +164,27 @@ int setops_gen_uneven_fill(uint64_t seed, uint64_t npairs, double lmin, double l
 // Skewed batch: one large set of `large_n` distinct values from [0, universe), sorted, and
 // `nsmall` small sets of `small_n` values each: `shared` drawn (distinct) from the large set, the
 // rest from its complement; each small set sorted.  small[i*small_n ...] is set i.
+// Fixed-shape batch (the autotuner's grid cells): npairs pairs of `smaller` vs `larger` elements
+// over [0, universe) with `shared` planted in both; pair p is A[p*smaller ..] and B[p*larger ..].
+int setops_gen_fixed_batch(uint64_t seed, uint64_t npairs, uint64_t larger, uint64_t smaller, uint64_t universe,
+                           uint64_t shared, uint32_t *A, uint32_t *B)
+{
+    if (shared > smaller || smaller > larger || larger + smaller - shared > universe) return -1;
+    int bad = 0;
+#pragma omp parallel reduction(| : bad)
+    {
+        std::vector<uint8_t> seen(universe / 8 + 1, 0);
+        std::vector<uint32_t> scratch;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t p = 0; p < (int64_t)npairs; p++) {
+            SplitMix64 rng(stream_seed(seed, (uint64_t)p));
+            planted_pair(rng, universe, shared, smaller - shared, larger - shared, seen, scratch, A + p * smaller,
+                         B + p * larger);
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 int setops_gen_skewed(uint64_t seed, uint64_t universe, uint64_t large_n, uint64_t nsmall, uint64_t small_n,
                       uint64_t shared, uint32_t *large, uint32_t *small)
 {

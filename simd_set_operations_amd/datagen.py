@@ -80,6 +80,22 @@ def uneven_batch(npairs=UNEVEN["npairs"], seed=UNEVEN["seed"], lmin=UNEVEN["lmin
     return A, a_off, B, b_off
 
 
+def fixed_batch(npairs, larger, smaller, selectivity, density=0.1, seed=1006):
+    """Fixed-shape CSR batch: ``npairs`` pairs of ``smaller`` vs ``larger`` sorted unique u32 over
+    an element space of ``larger / density``, each with round(selectivity x smaller) shared
+    elements planted first.  Returns (A, a_off, B, b_off, shared)."""
+    universe = max(int(np.ceil(larger / density)), larger + smaller)
+    shared = int(round(selectivity * smaller))
+    A = np.zeros(npairs * smaller, np.uint32)
+    B = np.zeros(npairs * larger, np.uint32)
+    rc = _native.datagen().setops_gen_fixed_batch(seed, npairs, larger, smaller, universe, shared, _p(A), _p(B))
+    if rc != 0:
+        raise ValueError("fixed batch does not fit its element space")
+    a_off = np.arange(npairs + 1, dtype=np.uint64) * np.uint64(smaller)
+    b_off = np.arange(npairs + 1, dtype=np.uint64) * np.uint64(larger)
+    return A, a_off, B, b_off, shared
+
+
 def skewed(seed=SKEWED["seed"], universe=SKEWED["universe"], large=SKEWED["large"], nsmall=SKEWED["nsmall"],
            small=SKEWED["small"], shared=SKEWED["shared"]):
     """Skewed batch: (large[large], smalls[nsmall, small]) -- every small set has exactly `shared`

@@ -581,3 +581,43 @@ def test_datafile_pairs_through_every_entry(tmp_path):
     exp = oracle.branchless_merge(sa, sb)
     for name in PLAIN:
         assert np.array_equal(intersect.run_2set(sa, sb, name), exp), name
+
+
+def test_auto_entry_resolves_through_the_tuning_table():
+    """hip_auto batches take the entry the tuning table prices cheapest for their pair sizes;
+    their results are exact whichever entry that is."""
+    from simd_set_operations_amd import tuning
+    from simd_set_operations_amd.batch import DeviceBatch, GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(107))
+    As, Bs = [], []
+    for _ in range(600):
+        n = int(rng.integers(50, 3000))
+        As.append(sorted_set(rng, n, 8 * n))
+        Bs.append(sorted_set(rng, int(rng.integers(50, 3000)), 8 * n))
+    A, a_off = _csr(As)
+    B, b_off = _csr(Bs)
+    announce("hip_auto balanced CSR batch")
+    db = DeviceBatch("hip_auto", A, a_off, B, b_off)
+    assert db.algo == tuning.choose(np.diff(a_off), np.diff(b_off))
+    counts, off, data = _batch_expect(A, a_off, B, b_off)
+    db.run()
+    g_off, g_cnt, g_data = db.results()
+    assert np.array_equal(g_cnt, counts) and np.array_equal(g_off, off) and np.array_equal(g_data, data)
+    db.close()
+    # one resident large set against many small sets
+    L = sorted_set(rng, 1 << 20, 1 << 26)
+    smalls = [np.union1d(rng.choice(L, 4, replace=False), sorted_set(rng, 12, 1 << 26)).astype(np.uint32)
+              for _ in range(256)]
+    pool, s_off = _csr(smalls)
+    a_beg, a_end = s_off[:-1], s_off[1:]
+    b_beg = np.zeros(256, np.uint64)
+    b_end = np.full(256, L.size, np.uint64)
+    announce("hip_auto skewed gather batch")
+    gb = GatherBatch("hip_auto", pool, a_beg, a_end, L, b_beg, b_end)
+    assert gb.algo == tuning.choose(a_end - a_beg, b_end - b_beg)
+    gb.run()
+    _, cnt, _ = gb.results()
+    exp = [oracle.branchless_merge(s, L).size for s in smalls]
+    assert np.array_equal(cnt, np.array(exp, np.uint32))
+    gb.close()

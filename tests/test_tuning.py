@@ -1,0 +1,77 @@
+"""The autotuner's table and the cost model that picks an entry from it (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+from simd_set_operations_amd import autotune, datagen, tuning
+
+
+def _table(rates):
+    """A two-cell table: rates[(log2_larger, log2_skew)] = {algo: Gelem/s}."""
+    cells = []
+    for (l, s), g in rates.items():
+        for sel in (0.1, 0.9):
+            cells.append({"log2_larger": l, "log2_skew": s, "selectivity": sel, "gelems": dict(g)})
+    algos = sorted({a for g in rates.values() for a in g})
+    return {"algorithms": algos, "cells": cells}
+
+
+T = _table({(10, 0): {"m": 100.0, "g": 10.0}, (10, 10): {"m": 10.0, "g": 100.0}})
+
+
+def test_choose_follows_the_cheaper_entry_per_cell():
+    assert tuning.choose([1024] * 8, [1024] * 8, table=T) == "m"
+    assert tuning.choose([1] * 8, [1024] * 8, table=T) == "g"
+
+
+def test_predict_sums_elements_over_rates():
+    # one balanced pair (2048 elements) and one skewed pair (1025 elements)
+    p = tuning.predict([1024, 1], [1024, 1024], table=T)
+    assert math.isclose(p["m"], (2048 / 100e9 + 1025 / 10e9) * 1e3)
+    assert math.isclose(p["g"], (2048 / 10e9 + 1025 / 100e9) * 1e3)
+
+
+def test_mixed_batch_weighs_elements():
+    # many skewed pairs but most elements in one balanced pair: the balanced cell decides
+    a = [1] * 10 + [1024] * 4
+    b = [1024] * 14
+    pm = tuning.predict(a, b, table=T)
+    assert pm["m"] != pm["g"]
+    assert tuning.choose(a, b, table=T) == min(pm, key=pm.get)
+
+
+def test_committed_table_covers_the_grid():
+    t = tuning.load()
+    grid = t["grid"]
+    assert grid == autotune.GRID
+    keys = {(c["log2_larger"], c["log2_skew"], c["selectivity"]) for c in t["cells"]}
+    want = {(l, s, v) for l in grid["log2_larger"] for s in grid["log2_skew"] for v in grid["selectivity"]}
+    assert keys == want
+    for c in t["cells"]:
+        assert set(c["gelems"]) == set(t["algorithms"])
+        assert all(v > 0 for v in c["gelems"].values())
+        assert c["best"] == max(c["gelems"], key=c["gelems"].get)
+        larger, smaller, pairs = autotune.cell_shape(c["log2_larger"], c["log2_skew"])
+        assert (c["larger"], c["smaller"], c["pairs"]) == (larger, smaller, pairs)
+        assert c["elements"] == pairs * (larger + smaller)
+
+
+def test_committed_table_choices():
+    # the uneven batch is merge work; a 2^24 resident set against sets of 256 is search work
+    A, a_off, B, b_off = datagen.uneven_batch(npairs=4096)
+    assert tuning.choose(np.diff(a_off), np.diff(b_off)) == "hip_branchless_merge"
+    assert tuning.choose(np.full(4096, 256), np.full(4096, 1 << 24)) in ("hip_galloping", "hip_binary_search")
+
+
+@pytest.mark.parametrize("larger,smaller,sel", [(64, 64, 0.5), (1024, 8, 0.9), (256, 1, 1.0), (100, 30, 0.0)])
+def test_fixed_batch_shapes_and_planted_counts(larger, smaller, sel):
+    A, a_off, B, b_off, shared = datagen.fixed_batch(50, larger, smaller, sel)
+    assert shared == round(sel * smaller)
+    assert A.size == 50 * smaller and B.size == 50 * larger
+    for p in range(50):
+        a = A[a_off[p]:a_off[p + 1]].astype(np.int64)
+        b = B[b_off[p]:b_off[p + 1]].astype(np.int64)
+        assert np.all(np.diff(a) > 0) and np.all(np.diff(b) > 0)
+        assert b.max() < max(int(np.ceil(larger / 0.1)), larger + smaller)
+        assert np.intersect1d(a, b).size == shared
