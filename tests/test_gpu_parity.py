@@ -621,3 +621,83 @@ def test_auto_entry_resolves_through_the_tuning_table():
     exp = [oracle.branchless_merge(s, L).size for s in smalls]
     assert np.array_equal(cnt, np.array(exp, np.uint32))
     gb.close()
+
+
+def test_experiment_harness_on_gpu_entries(tmp_path):
+    """The reference's experiment flow (generate, then time every algorithm per x) over the GPU
+    entries: reference names resolve to them, every datafile is timed, and each entry's result
+    count on every datafile equals the oracle's."""
+    from simd_set_operations_amd import datafile
+    from simd_set_operations_amd import experiment as ex
+
+    toml = tmp_path / "e.toml"
+    toml.write_text("""
+[algorithm_sets]
+all = ["branchless_merge", "galloping", "binary_search", "shuffling_avx2", "qfilter", "branchless_merge_bsr",
+       "croaring"]
+[[experiment]]
+name = "two"
+title = "2-set skew"
+dataset = "two"
+algorithm_set = "all"
+[[experiment]]
+name = "four"
+title = "4-set"
+dataset = "four"
+algorithms = ["branchless_merge", "galloping", "shuffling_sse"]
+[[dataset]]
+name = "two"
+type = "synthetic"
+set_count = 2
+gen_count = 2
+vary = "skew"
+skewness_factor = 0
+to = 4000
+step = 2000
+density = 10
+selectivity = 300
+max_len = 12
+[[dataset]]
+name = "four"
+type = "synthetic"
+set_count = 4
+gen_count = 2
+vary = "set_count"
+set_count_unused = 0
+skewness_factor = 1000
+to = 5
+step = 1
+density = 50
+selectivity = 500
+max_len = 11
+""".replace("set_count_unused = 0\n", ""))
+    doc = ex.load(toml)
+    d = str(tmp_path / "datasets")
+    ex.generate(doc, d, log=lambda m: None)
+    announce("experiment harness: run")
+    res = ex.run(doc, d, resolver=ex.gpu_resolver(), warmup_s=0.05, log=lambda m: None)
+    two = res["datasets"]["two"]["algos"]
+    assert two["croaring"] == []  # no GPU entry: skipped, as the reference skips unknown names
+    for name in ("branchless_merge", "galloping", "binary_search", "shuffling_avx2", "qfilter", "branchless_merge_bsr"):
+        assert [r["x"] for r in two[name]] == [0, 2000, 4000]
+        assert all(len(r["times"]) == 2 and min(r["times"]) > 0 for r in two[name])
+    four = res["datasets"]["four"]["algos"]
+    assert [r["x"] for r in four["galloping"]] == [4, 5]
+    resolve = ex.gpu_resolver()
+    resolve_co = ex.gpu_resolver(count_only=True)
+    for ds in ("two", "four"):
+        for x in ex.xvalues(res["datasets"][ds]["info"]):
+            for i in range(2):
+                sets = datafile.read(os.path.join(d, ds, str(x), str(i)))
+                exp = oracle.svs("branchless_merge", sets).size
+                for name in res["datasets"][ds]["algos"]:
+                    t = resolve(name)
+                    if t is None or (t.bsr and len(sets) > 2):
+                        continue
+                    assert t.run(t.prepare(sets)) == exp, (ds, x, i, name)
+                    tc = resolve_co(name)
+                    assert tc.run(tc.prepare(sets)) == exp, (ds, x, i, name, "count-only")
+    # device-resident 2-set timing
+    res = ex.run(doc, d, experiments=["two"], resolver=ex.gpu_resolver(resident=True), warmup_s=0.02,
+                 log=lambda m: None)
+    assert all(len(r["times"]) == 2 for r in res["datasets"]["two"]["algos"]["galloping"])
