@@ -100,7 +100,11 @@ int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t 
 /* ---- k-set, host pointers ------------------------------------------------------------------
  * Replaces svs_generic (setops/src/intersect/svs.rs:50-73) driven by a 2-set entry, as timed by
  * time_svs (benchmark/src/timer/harness.rs:152-172): sets[0] ∩ sets[1], then the running result
- * ∩ sets[2], ... in the order given (the caller orders smallest-first).  k >= 2. */
+ * ∩ sets[2], ... in the order given (the caller orders smallest-first).  k >= 2.
+ * With "hip_small_adaptive" the same call replaces small_adaptive (setops/src/intersect/
+ * adaptive.rs:107-147, the "small_adaptive" k-set entry of benchmark/src/timer.rs): every element
+ * of sets[0] is looked up in sets[1], sets[2], ... in turn, in one launch per 1 + 8 further sets,
+ * with no intermediate result between the sets of a launch. */
 int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, size_t k, uint32_t *out,
                    size_t out_cap, size_t *out_len);
 

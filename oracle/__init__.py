@@ -90,6 +90,8 @@ def lib():
         L.oracle_intersect2.argtypes = [_c.c_int, _vp, _sz, _vp, _sz, _vp]
         L.oracle_svs.restype = _sz
         L.oracle_svs.argtypes = [_c.c_int, _vp, _vp, _sz, _vp, _vp, _c.POINTER(_vp)]
+        L.oracle_small_adaptive.restype = _sz
+        L.oracle_small_adaptive.argtypes = [_vp, _vp, _sz, _vp, _vp]
         L.oracle_batch_intersect.restype = _c.c_int
         L.oracle_batch_intersect.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]
         L.oracle_batch_intersect_gather.restype = _c.c_int
@@ -192,6 +194,18 @@ def svs(algo, sets):
     n = lib().oracle_svs(code, _c.cast(ptrs, _vp), _c.cast(lens, _vp), k, _p(left), _p(right), _c.byref(res))
     which = left if res.value == left.ctypes.data else right
     return which[:n].copy()
+
+
+def small_adaptive(sets):
+    """adaptive.rs:107-147 over the sets in the given order; returns the visited values."""
+    arrs = [_u32(s) for s in sets]
+    k = len(arrs)
+    out = np.zeros(max(arrs[0].size, 1), np.uint32)
+    pos = (_sz * k)()
+    ptrs = (_vp * k)(*[a.ctypes.data if a.size else None for a in arrs])
+    lens = (_sz * k)(*[a.size for a in arrs])
+    n = lib().oracle_small_adaptive(_c.cast(ptrs, _vp), _c.cast(lens, _vp), k, _c.cast(pos, _vp), _p(out))
+    return out[:n].copy()
 
 
 def batch_intersect(algo, A, a_off, B, b_off, nthreads=1, with_output=True):

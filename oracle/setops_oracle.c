@@ -23,6 +23,7 @@
  *   oracle_bsr_to_sorted          setops/src/bsr.rs:77-88
  *   oracle_bsr_count              setops/src/visitor.rs:399-403 (Counter::visit_bsr = popcount)
  *   oracle_svs                    setops/src/intersect/svs.rs:50-73 (svs_generic fold)
+ *   oracle_small_adaptive         setops/src/intersect/adaptive.rs:107-147
  *
  * Values are compared as uint32_t.  The reference's harness feeds i32 (benchmark/src/datafile.rs:25);
  * for values below 2^31 -- every workload here -- the two orders coincide.
@@ -445,6 +446,35 @@ size_t oracle_svs(int algo, const uint32_t *const *sets, const size_t *lens, siz
     }
     if (result) *result = left;
     return n;
+}
+
+/* adaptive.rs:107-147 (Demaine, Lopez-Ortiz, Munro): for each element of sets[0], in order, look it
+ * up in sets[1], sets[2], ... in turn -- double an offset from the set's remembered position while
+ * set[pos + offset] <= element, binary-search [pos, min(len-1, pos+offset)], remember the found
+ * position -- and move to the next element at the first set that lacks it; an element found in
+ * every set is emitted.  `pos` needs k entries of scratch. */
+size_t oracle_small_adaptive(const uint32_t *const *sets, const size_t *lens, size_t k, size_t *pos, uint32_t *out)
+{
+    size_t count = 0;
+    if (k < 2) return (size_t)-1;
+    for (size_t i = 0; i < k; i++) pos[i] = 0;
+    for (size_t e = 0; e < lens[0]; e++) {
+        const uint32_t element = sets[0][e];
+        int in_all = 1;
+        for (size_t i = 1; i < k && in_all; i++) {
+            const uint32_t *set = sets[i];
+            const size_t len = lens[i], base = pos[i];
+            size_t offset = 1;
+            while (base + offset < len && set[base + offset] <= element) offset *= 2;
+            int64_t hi = (int64_t)len - 1;
+            if ((int64_t)(base + offset) < hi) hi = (int64_t)(base + offset);
+            const size_t nb = oracle_binary_search(set, element, (int64_t)base, hi);
+            pos[i] = nb;
+            if (nb >= len || set[nb] != element) in_all = 0;
+        }
+        if (in_all) EMIT(element);
+    }
+    return count;
 }
 
 /* ------------------------------------------------------------------------------------------------

@@ -45,6 +45,8 @@ struct SearchSmem {
     uint32_t win_c[MAXSEG + 1];
     uint64_t win_g[MAXSEG];
     uint32_t win_src[MAXSEG];
+    uint32_t xlo[kMaxExtraSets];  // further sets (single pair): the tile's probe range in each
+    uint32_t xhi[kMaxExtraSets];
 };
 
 constexpr uint32_t kNotStaged = 0xFFFFFFFFu;
@@ -233,6 +235,31 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             __syncthreads();
         }
 
+        // ---- 3b. further sets (a single pair, plain sets: small_adaptive over k sets) -----------------
+        // Every probe found in the pair's larger set is also looked up in each further set in turn
+        // (adaptive.rs:107-147); each further set is narrowed to the tile's probe range here, one
+        // wave per set, and each lane gallops forward through it from there.
+        uint32_t xcur[kMaxExtraSets];
+        const uint32_t nx = (!BSR && nseg == 1 && sm.seg_wa[0] > 0) ? in.nx : 0u;
+        if (nx) {
+            const uint32_t vf = sm.pkey[sm.seg_la[0]], vl = sm.pkey[sm.seg_la[0] + sm.seg_wa[0] - 1];
+            for (uint32_t j = wave; j < nx; j += NW) {
+                const uint32_t *__restrict__ X = in.xsets[j];
+                const uint64_t nxj = in.xlens[j];
+                const uint32_t lo =
+                    (uint32_t)group_partition_point<kWave>(0, nxj, lane, 0, [&](uint64_t i) { return X[i] < vf; });
+                const uint32_t hi =
+                    (uint32_t)group_partition_point<kWave>(lo, nxj, lane, 0, [&](uint64_t i) { return X[i] <= vl; });
+                if (lane == 0) {
+                    sm.xlo[j] = lo;
+                    sm.xhi[j] = hi;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)kMaxExtraSets; j++) xcur[j] = (j < nx) ? sm.xlo[j] : 0u;
+        }
+
         // ---- 4. lookups --------------------------------------------------------------------------
         const uint32_t ub = tid * KS;
         uint32_t hmask = 0, nelems = 0;
@@ -286,6 +313,17 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                         if (found) lstate = (large_is_a ? in.As : in.Bs)[sm.seg_lbase[s] + pos];
                 }
                 cur = pos;
+                if constexpr (!BSR) {
+                    // further sets in order; the first one that lacks x ends the lookup
+                    for (uint32_t j = 0; found && j < nx; j++) {
+                        const uint32_t *__restrict__ X = in.xsets[j];
+                        auto xarr = [&](uint32_t i) { return X[i]; };
+                        const uint32_t xh = sm.xhi[j];
+                        const uint32_t xp = lower_bound_from<GALLOP>(xarr, xcur[j], xh, x);
+                        xcur[j] = xp;
+                        found = xp < xh && X[xp] == x;
+                    }
+                }
                 if constexpr (BSR) {
                     const uint32_t stv = found ? (sm.pst[sm.seg_la[s] + e] & lstate) : 0u;
                     found = stv != 0;

@@ -68,7 +68,7 @@ constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this ma
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
 constexpr int kPlaceQ = 8;  // words per lane per round of the placement kernel
 
-enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4 };
+enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4, KIND_ADAPTIVE = 5 };
 
 struct AlgoEntry {
     const char *name;
@@ -85,6 +85,9 @@ const AlgoEntry kAlgos[] = {
     {"hip_galloping", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_GALLOP, false},
     {"hip_galloping_bsr", SETOPS_ALGO_BSR, KIND_GALLOP, true},
     {"hip_binary_search", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_BINARY, false},
+    // small_adaptive (adaptive.rs:107-147): k sets in one pass, each element of the first set
+    // galloped through the others in turn; on two sets it is galloping
+    {"hip_small_adaptive", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_ADAPTIVE, false},
     {"hip_shuffling", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_SHUFFLE, false},
     {"hip_shuffling_bsr", SETOPS_ALGO_BSR, KIND_SHUFFLE, true},
     {"hip_qfilter", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_QFILTER, false},
@@ -389,7 +392,8 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
         if (h->gather) launch_merge_t<BSR, CO, true, LP_QFILTER>(h, out, stream);
         else launch_merge_t<BSR, CO, false, LP_QFILTER>(h, out, stream);
         break;
-    case KIND_GALLOP: launch_search_t<BSR, CO, true>(h, out, stream); break;
+    case KIND_GALLOP:
+    case KIND_ADAPTIVE: launch_search_t<BSR, CO, true>(h, out, stream); break;
     case KIND_BINARY: launch_search_t<BSR, CO, false>(h, out, stream); break;
     }
 }
@@ -638,7 +642,7 @@ int host_scratch(size_t bytes, void **p)
 // and are written by the plan kernel, so a single-pair run needs no host-to-device copy of sizes.
 int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, const uint32_t *dB,
                   const uint32_t *dBs, size_t nb, uint32_t *dO, uint32_t *dOs, size_t cap, unsigned long long *dTot,
-                  hipStream_t s)
+                  hipStream_t s, const uint32_t *const *dX = nullptr, const uint64_t *dXlen = nullptr, uint32_t nx = 0)
 {
     const AlgoEntry *e = algo_entry(algo);
     setops_hip_batch *&h = g_ctx.single[algo];
@@ -667,6 +671,9 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     h->in.b_pool = nb;
     h->in.P = 1;
     h->in.contig = 1;
+    h->in.xsets = dX;
+    h->in.xlens = dXlen;
+    h->in.nx = nx;
     h->plan.single_na = na;
     h->plan.single_nb = nb;
     return batch_launch(h, dO, dOs, cap, nullptr, nullptr, dTot, s);
@@ -726,6 +733,83 @@ int run_single_pair(int algo, const uint32_t *a, const uint32_t *as, size_t na, 
             HIP_TRY(hipMemcpy(out, dO, n * 4, hipMemcpyDeviceToHost));
             if (e->bsr) HIP_TRY(hipMemcpy(out_s, dOs, n * 4, hipMemcpyDeviceToHost));
         }
+    }
+    return SETOPS_OK;
+}
+
+
+// small_adaptive over k > 2 host sets (adaptive.rs:107-147): every set is uploaded once; one
+// launch intersects the first set with the next one and up to kMaxExtraSets further sets (each
+// probe found in the second set is looked up in each further set in turn); with more sets the
+// result, still on the device, becomes the first set of the next launch.
+int run_adaptive(int algo, const uint32_t *const *sets, const size_t *lens, size_t k, uint32_t *out, size_t out_cap,
+                 size_t *out_len)
+{
+    const size_t groups = (k - 1 + kMaxExtraSets) / (kMaxExtraSets + 1);  // launches of 1 + kMaxExtraSets sets
+    const size_t table = align_up(groups * kMaxExtraSets * 8, 256);      // pointers, then lengths
+    size_t bytes = 256 + 2 * table;
+    for (size_t s = 0; s < k; s++) bytes += align_up(lens[s] * 4 + 16, 256);
+    const size_t cap = std::min(lens[0], lens[1]);  // bounds every launch's result
+    const size_t res_bytes = align_up(std::max<size_t>(cap, 1) * 4 + 16, 256);
+    bytes += 2 * res_bytes;
+    void *buf = nullptr;
+    int rc = host_scratch(bytes, &buf);
+    if (rc != SETOPS_OK) return rc;
+    char *p = static_cast<char *>(buf);
+    hipStream_t st = g_ctx.stream;
+    unsigned long long *dTot = reinterpret_cast<unsigned long long *>(p);
+    p += 256;
+    std::vector<const uint32_t *> dsets(k);
+    for (size_t s = 0; s < k; s++) {
+        uint32_t *d = reinterpret_cast<uint32_t *>(p);
+        if (lens[s]) HIP_TRY(hipMemcpyAsync(d, sets[s], lens[s] * 4, hipMemcpyHostToDevice, st));
+        dsets[s] = d;
+        p += align_up(lens[s] * 4 + 16, 256);
+    }
+    uint32_t *res[2] = {reinterpret_cast<uint32_t *>(p), reinterpret_cast<uint32_t *>(p + res_bytes)};
+    p += 2 * res_bytes;
+    // pointer and length tables of every launch's further sets
+    const uint32_t *const *dX = reinterpret_cast<const uint32_t *const *>(p);
+    p += table;
+    const uint64_t *dL = reinterpret_cast<const uint64_t *>(p);
+    p += table;
+    // launch g intersects `first` (sets[0], then the previous result) with sets[next] and the
+    // further sets sets[next+1 .. next+nx]
+    const uint32_t *first = dsets[0];
+    size_t nfirst = lens[0];
+    size_t next = 1;
+    int which = 0;
+    for (size_t g = 0; next < k; g++) {
+        if (g >= groups) return fail(SETOPS_ERR_INVALID, "internal: launch count");
+        const size_t nx = std::min<size_t>(kMaxExtraSets, k - next - 1);
+        // the further sets of this launch are sets[next+1 .. next+nx]
+        std::vector<const uint32_t *> gx(kMaxExtraSets, nullptr);
+        std::vector<uint64_t> gl(kMaxExtraSets, 0);
+        for (size_t j = 0; j < nx; j++) {
+            gx[j] = dsets[next + 1 + j];
+            gl[j] = lens[next + 1 + j];
+        }
+        const uint32_t *const *dXg = dX + g * kMaxExtraSets;
+        const uint64_t *dLg = dL + g * kMaxExtraSets;
+        HIP_TRY(hipMemcpyAsync(const_cast<const uint32_t **>(dXg), gx.data(), kMaxExtraSets * sizeof(void *),
+                               hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(const_cast<uint64_t *>(dLg), gl.data(), kMaxExtraSets * 8, hipMemcpyHostToDevice, st));
+        rc = single_launch(algo, first, nullptr, nfirst, dsets[next], nullptr, lens[next], res[which], nullptr,
+                           std::max<size_t>(cap, 1), dTot, st, dXg, dLg, (uint32_t)nx);
+        if (rc != SETOPS_OK) return rc;
+        unsigned long long tot[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(tot, dTot, sizeof tot, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        first = res[which];
+        nfirst = (size_t)tot[0];
+        which ^= 1;
+        next += 1 + nx;
+    }
+    *out_len = nfirst;
+    if (out) {
+        if (nfirst > out_cap)
+            return fail(SETOPS_ERR_CAPACITY, "result has %zu elements, output capacity is %zu", nfirst, out_cap);
+        if (nfirst) HIP_TRY(hipMemcpy(out, first, nfirst * 4, hipMemcpyDeviceToHost));
     }
     return SETOPS_OK;
 }
@@ -832,6 +916,7 @@ int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, si
         if (lens[s] && !sets[s]) return fail(SETOPS_ERR_INVALID, "set %zu is NULL", s);
     ENSURE_INIT();
     std::lock_guard<std::mutex> lk(g_ctx.host_mu);
+    if (e->kind == KIND_ADAPTIVE && k > 2) return run_adaptive(algo, sets, lens, k, out, out_cap, out_len);
     // svs.rs:50-73: fold the 2-set entry over the sets in the given order.
     std::vector<uint32_t> left(std::max<size_t>(1, std::min(lens[0], lens[1]))), right;
     size_t n = 0;

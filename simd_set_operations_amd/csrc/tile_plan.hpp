@@ -42,7 +42,14 @@ struct PairBatch {
     uint64_t P;
     uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
     const uint32_t *b_idx;  // optional: b_idx[k] = B[32k] (search policy: resident B pools)
+    // optional, one pair, search policy, plain sets: nx further sets (device pointers and
+    // lengths); a probe's result needs it in every one of them as well
+    const uint32_t *const *xsets;
+    const uint64_t *xlens;
+    uint32_t nx;
 };
+
+constexpr int kMaxExtraSets = 8;  // further sets per launch (k-set queries run in groups of 2 + 8)
 
 // Index of a resident pool: idx[k] = pool[32k], k < ceil(n / 32).
 __global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx)

@@ -14,7 +14,8 @@ from simd_set_operations_amd.visitor import Counter, VecWriter
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
-PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search", "hip_shuffling", "hip_qfilter"]
+PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search", "hip_shuffling", "hip_qfilter",
+         "hip_small_adaptive"]
 BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr"]
 
 
@@ -701,3 +702,24 @@ max_len = 11
     res = ex.run(doc, d, experiments=["two"], resolver=ex.gpu_resolver(resident=True), warmup_s=0.02,
                  log=lambda m: None)
     assert all(len(r["times"]) == 2 for r in res["datasets"]["two"]["algos"]["galloping"])
+
+
+def test_small_adaptive_many_sets():
+    """hip_small_adaptive on up to 21 sets (three launches of 1 + 8 further sets), in any order,
+    with empty and repeated sets: equals the oracle's small_adaptive and the k-way intersection."""
+    rng = np.random.Generator(np.random.PCG64(109))
+    for trial in range(24):
+        k = int(rng.integers(3, 22))
+        common = sorted_set(rng, int(rng.integers(0, 400)), 1 << 22)
+        sets = [np.union1d(sorted_set(rng, int(rng.integers(0, 60000)), 1 << 22), common).astype(np.uint32)
+                for _ in range(k)]
+        if trial % 2:
+            sets.sort(key=len)
+        if trial % 5 == 0:
+            sets[int(rng.integers(0, k))] = np.zeros(0, np.uint32)
+        if trial % 3 == 0:
+            sets[-1] = sets[0].copy()
+        announce(f"hip_small_adaptive k={k} trial={trial}")
+        exp = oracle.small_adaptive(sets)
+        assert np.array_equal(exp, oracle.svs("branchless_merge", sets))
+        assert np.array_equal(intersect.run_kset(sets, "hip_small_adaptive"), exp), trial

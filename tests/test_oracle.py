@@ -207,6 +207,36 @@ def test_svs_fold(algo):
         assert prop_intersection_correct(got, sets)
 
 
+def test_small_adaptive():
+    """adaptive.rs:107-147: equals the k-way intersection (numpy) in any set order, satisfies the
+    invariant trio, handles empty and identical sets; brute force over a tiny universe."""
+    rng = np.random.Generator(np.random.PCG64(17))
+    for trial in range(60):
+        k = int(rng.integers(2, 12))
+        common = sorted_set(rng, int(rng.integers(0, 40)), 3000)
+        sets = [np.union1d(sorted_set(rng, int(rng.integers(0, 1500)), 3000), common).astype(np.uint32)
+                for _ in range(k)]
+        if trial % 2:
+            sets.sort(key=len)
+        if trial % 7 == 0:
+            sets[int(rng.integers(0, k))] = np.zeros(0, np.uint32)
+        if trial % 5 == 0:
+            sets[-1] = sets[0].copy()
+        exp = sets[0]
+        for s in sets[1:]:
+            exp = np.intersect1d(exp, s)
+        got = oracle.small_adaptive(sets)
+        assert np.array_equal(got, exp.astype(np.uint32)), trial
+        assert prop_intersection_correct(got, sets)
+    # brute force: every 3-subset pattern of a 12-value universe
+    for mask in range(1 << 12):
+        a = np.array([v for v in range(12) if mask >> v & 1], np.uint32)
+        b = np.array([v for v in range(12) if (mask * 7 + 3) >> v & 1], np.uint32)
+        c = np.array([v for v in range(12) if v % 3 != mask % 3], np.uint32)
+        want = np.array([v for v in range(12) if v in a and v in b and v in c], np.uint32)
+        assert np.array_equal(oracle.small_adaptive([a, b, c]), want), mask
+
+
 def test_batch_runner_matches_pairwise():
     rng = np.random.Generator(np.random.PCG64(17))
     P = 200
