@@ -299,6 +299,50 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     return res
 
 
+def kset_leg(with_cpu):
+    """k-set queries through the host-pointer k-set call (the plain drop-in for svs_generic and
+    small_adaptive): sets generated as the reference's gen_kset (generators.rs:113-135) with
+    max_len 2^18, skew factor 1 (set i has 2^18 / (i+1) elements), density 1%, selectivity 10%.
+    Per query: median of 5 calls (upload, launches, download), svs over merge / galloping and
+    small_adaptive (one launch per 1 + 8 sets, no intermediate round trips)."""
+    from simd_set_operations_amd import experiment, intersect
+
+    res = {}
+    for k in (3, 5, 8):
+        props = {"set_count": k, "density": 10, "selectivity": 100, "max_len": 18, "skewness_factor": 1000}
+        queries = [experiment.gen_kset(props, np.random.Generator(np.random.PCG64([77, k, i]))) for i in range(3)]
+        r = {"elements_per_query": int(sum(s.size for s in queries[0]))}
+        for algo in ("hip_branchless_merge", "hip_galloping", "hip_small_adaptive"):
+            times = []
+            for sets in queries:
+                exp = intersect.run_kset(sets, algo)  # warm
+                ts = []
+                for _ in range(5):
+                    t0 = time.perf_counter()
+                    got = intersect.run_kset(sets, algo)
+                    ts.append(time.perf_counter() - t0)
+                if got.size != exp.size:
+                    log(f"kset {algo}: result changed between calls")
+                    sys.exit(3)
+                times.append(float(np.median(ts)) * 1e3)
+            r[algo] = {"ms_per_query": float(np.mean(times))}
+        if with_cpu:
+            import oracle
+
+            for name, fn in (("cpu_port_small_adaptive", oracle.small_adaptive),
+                             ("cpu_port_svs_branchless_merge", lambda q: oracle.svs("branchless_merge", q))):
+                t0 = time.perf_counter()
+                for sets in queries:
+                    got = fn(sets)
+                    if not np.array_equal(got, intersect.run_kset(sets, "hip_small_adaptive")):
+                        log(f"kset: {name} differs from hip_small_adaptive")
+                        sys.exit(3)
+                r[name] = {"ms_per_query": (time.perf_counter() - t0) / len(queries) * 1e3, "threads": 1}
+        res[f"k{k}"] = r
+        log(f"kset k={k}: {r}")
+    return res
+
+
 def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
     """The served k-set workload: a device-resident corpus of 2^20 sorted lists (bounded-Zipf(1.1)
     lengths in [2^4, 2^18] over [0, 2^26)); queries of k ~ U{2..6} lists by Zipf(1.0) popularity,
@@ -356,6 +400,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the R-MAT graph leg")
     ap.add_argument("--graph-scale", type=int, default=22)
     ap.add_argument("--no-serving", action="store_true", help="skip the served k-set leg")
+    ap.add_argument("--no-kset", action="store_true", help="skip the host-pointer k-set leg")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
     ap.add_argument("--corpus-mode", choices=("iid", "rank"), default="iid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -482,6 +527,8 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_kset:
+        extra["kset"] = kset_leg(not args.no_baselines)
     if not args.no_serving:
         extra["serving"] = serving_leg(torch, (5e4, 2e5, 8e5), args.serving_seconds, 5.0, args.corpus_mode)
     if not args.no_graph:
