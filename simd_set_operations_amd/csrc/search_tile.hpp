@@ -47,9 +47,13 @@ struct SearchSmem {
     uint32_t win_src[MAXSEG];
     uint32_t xlo[kMaxExtraSets];  // further sets (single pair): the tile's probe range in each
     uint32_t xhi[kMaxExtraSets];
+    // top level of the B-pool index over the tile's larger B sets: top[j - top_j0] = B[kTopStride j]
+    alignas(16) uint32_t top[kTopCap];
+    unsigned long long red64[NT / kWave][2];
 };
 
 constexpr uint32_t kNotStaged = 0xFFFFFFFFu;
+constexpr uint32_t kNoNarrow = 16 * 3072;  // larger sets searched through the B-pool index unnarrowed
 
 // Lower bound of x in arr[lo, hi) starting from lo: galloping (probe lo+1, lo+3, lo+7, ... then
 // binary search in the last gap) or plain binary search.  Works on LDS or global arrays.
@@ -92,6 +96,46 @@ __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, 
     }
     if (lo == 0) return kNoPos;  // kb == 0 and x < B[0] = B[g0]
     const uint64_t base = 32 * (lo - 1);  // the line [base, base+32) holds every candidate
+    uint64_t hit = kNoPos;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint64_t e = base + 4ull * c;
+        const uint4 v = load_chunk(B, pool, e);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t q = e + (uint64_t)j;
+            if (q >= g0 && q < g1 && q < pool && w[j] == x) hit = q;
+        }
+    }
+    return hit;
+}
+
+// index_probe with a top level of the index in LDS: top[j - tj0] = idx[kTopIdx j] (= B[kTopStride j])
+// for every j whose kTopIdx*j lies in the probe's entry range.  The LDS search narrows the index
+// search to kTopIdx entries, so a probe costs log2(kTopIdx) dependent index reads (L2) instead of
+// log2 of the whole range, then one 128-byte line of B as in index_probe.
+__device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__ B, uint64_t pool,
+                                                    const uint32_t *__restrict__ idx, const uint32_t *top,
+                                                    uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x)
+{
+    if (g0 >= g1) return kNoPos;
+    const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;                // entries with 32k in [g0, g1)
+    const uint64_t jb = (kb + kTopIdx - 1) / kTopIdx, je = (ke + kTopIdx - 1) / kTopIdx;  // top entries inside
+    uint64_t lo = jb, hi = je;  // first top entry > x
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (top[mid - tj0] <= x) lo = mid + 1; else hi = mid;
+    }
+    // the first index entry > x lies in [L, H]: idx[kTopIdx (j-1)] <= x < idx[kTopIdx j] for j = lo
+    uint64_t L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
+    uint64_t H = (lo < je) ? kTopIdx * lo : ke;
+    while (L < H) {
+        const uint64_t mid = (L + H) >> 1;
+        if (idx[mid] <= x) L = mid + 1; else H = mid;
+    }
+    if (L == 0) return kNoPos;  // x < B[0]
+    const uint64_t base = 32 * (L - 1);  // the line [base, base+32) holds every candidate
     uint64_t hit = kNoPos;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -171,12 +215,56 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
         }
         __syncthreads();
 
+        // ---- 1b. top level of the B-pool index over the tile's larger B sets ------------------------
+        bool top_ok = false;
+        uint64_t tj0 = 0;
+        if (in.b_top) {
+            unsigned long long gmin = ~0ull, gmax = 0;
+            if (tid < nseg && !((sm.seg_flags[tid] >> 2) & 1u) && sm.seg_hi[tid] > 0) {
+                gmin = sm.seg_lbase[tid];
+                gmax = sm.seg_lbase[tid] + sm.seg_hi[tid];
+            }
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                gmin = min(gmin, (unsigned long long)__shfl_xor(gmin, d, kWave));
+                gmax = max(gmax, (unsigned long long)__shfl_xor(gmax, d, kWave));
+            }
+            if (lane == 0) {
+                sm.red64[wave][0] = gmin;
+                sm.red64[wave][1] = gmax;
+            }
+            __syncthreads();
+            gmin = ~0ull;
+            gmax = 0;
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                gmin = min(gmin, sm.red64[k][0]);
+                gmax = max(gmax, sm.red64[k][1]);
+            }
+            if (gmax > gmin) {
+                // entries j with kTopStride j in [gmin, gmax)
+                tj0 = (gmin + kTopStride - 1) / kTopStride;
+                const uint64_t tn = (gmax + kTopStride - 1) / kTopStride - tj0;
+                if (tn <= (uint64_t)kTopCap) {
+                    top_ok = true;
+                    for (uint32_t i = tid; i < (uint32_t)tn; i += NT) sm.top[i] = in.b_top[tj0 + i];
+                }
+            }
+            __syncthreads();
+        }
+
         // ---- 2. narrow the larger sets: one wave per segment ----------------------------------------
+        // A larger set in an indexed B pool longer than kNoNarrow elements is not narrowed: its
+        // window could be staged only if the probes clustered, and the two 64-way searches would
+        // cost more HBM lines than the probes themselves; every probe finds its line through the
+        // index instead.
         for (uint32_t s = wave; s < nseg; s += NW) {
             const uint32_t n = sm.seg_wa[s];
             const uint32_t nl = sm.seg_hi[s];
             uint32_t lo = 0, hi = 0;
-            if (n) {
+            if (n && in.b_idx && !((sm.seg_flags[s] >> 2) & 1u) && nl > kNoNarrow) {
+                hi = nl;
+            } else if (n) {
                 const bool large_is_a = (sm.seg_flags[s] >> 2) & 1u;
                 const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
                 const uint32_t vf = sm.pkey[sm.seg_la[s]], vl = sm.pkey[sm.seg_la[s] + n - 1];
@@ -299,7 +387,9 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     // resident B pool with an index of every 32nd element: binary search the index,
                     // then compare one 128-byte line of B
                     const uint64_t g = sm.seg_lbase[s];
-                    const uint64_t q = index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x);
+                    const uint64_t q = top_ok
+                        ? index_probe_top(in.B, in.b_pool, in.b_idx, sm.top, tj0, g + cur, g + hi_s, x)
+                        : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x);
                     found = q != kNoPos;
                     pos = found ? (uint32_t)(q - g) : cur;
                     if constexpr (BSR)

@@ -171,7 +171,8 @@ struct setops_hip_batch {
     uint64_t cap_tiles = 0;
     uint32_t *d_error = nullptr;
     uint32_t *stage = nullptr;  // result staging (separate allocation, see stage_reserve)
-    uint32_t *b_idx = nullptr;  // index of the B pool (setops_hip_batch_index_b)
+    uint32_t *b_idx = nullptr;  // index of the B pool (setops_hip_batch_index_b): every 32nd element
+    uint32_t *b_top = nullptr;  // its top level: every kTopStride-th element
     uint64_t stage_tiles = 0;
     unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
     hipStream_t last_stream = nullptr;
@@ -614,6 +615,7 @@ void batch_free(setops_hip_batch *h)
     if (h->ws) (void)hipFree(h->ws);
     if (h->stage) (void)hipFree(h->stage);
     if (h->b_idx) (void)hipFree(h->b_idx);
+    if (h->b_top) (void)hipFree(h->b_top);
     delete h;
 }
 
@@ -966,18 +968,30 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
     if (handle->b_idx) return SETOPS_OK;
     if (merge_like(handle->kind)) return SETOPS_OK;  // only the search policy reads an index
     const uint64_t m = (handle->in.b_pool + 31) / 32;
+    const uint64_t mt = (handle->in.b_pool + kTopStride - 1) / kTopStride;
     if (m == 0) return SETOPS_OK;
     if (hipMalloc(&handle->b_idx, m * 4) != hipSuccess) {
         (void)hipGetLastError();
         handle->b_idx = nullptr;
         return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)(m * 4));
     }
+    if (hipMalloc(&handle->b_top, mt * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(handle->b_idx);
+        handle->b_idx = nullptr;
+        handle->b_top = nullptr;
+        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)(mt * 4));
+    }
     hipStream_t s = handle->last_stream ? handle->last_stream : g_ctx.stream;
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
-                       handle->b_idx);
+                       handle->b_idx, (uint64_t)32);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(mt, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
+                       handle->b_top, kTopStride);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     handle->in.b_idx = handle->b_idx;
+    handle->in.b_top = handle->b_top;
     return SETOPS_OK;
 }
 

@@ -42,6 +42,7 @@ struct PairBatch {
     uint64_t P;
     uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
     const uint32_t *b_idx;  // optional: b_idx[k] = B[32k] (search policy: resident B pools)
+    const uint32_t *b_top;  // optional, with b_idx: b_top[j] = B[kTopStride j] (staged in LDS per tile)
     // optional, one pair, search policy, plain sets: nx further sets (device pointers and
     // lengths); a probe's result needs it in every one of them as well
     const uint32_t *const *xsets;
@@ -50,13 +51,17 @@ struct PairBatch {
 };
 
 constexpr int kMaxExtraSets = 8;  // further sets per launch (k-set queries run in groups of 2 + 8)
+constexpr uint64_t kTopIdx = 256;              // index entries per top-level entry
+constexpr uint64_t kTopStride = 32 * kTopIdx;  // pool elements per top-level entry (8,192)
+constexpr int kTopCap = 2048;                  // top-level entries a search tile stages (8 KB of LDS)
 
-// Index of a resident pool: idx[k] = pool[32k], k < ceil(n / 32).
-__global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx)
+// Index of a resident pool: idx[k] = pool[stride * k], k < ceil(n / stride).
+__global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx,
+                                   uint64_t stride)
 {
-    const uint64_t m = (n + 31) / 32;
+    const uint64_t m = (n + stride - 1) / stride;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
-        idx[k] = pool[32 * k];
+        idx[k] = pool[stride * k];
 }
 
 enum : uint32_t { POLICY_MERGE = 0, POLICY_SEARCH = 1 };
