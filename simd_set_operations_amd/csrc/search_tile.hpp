@@ -112,11 +112,36 @@ __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, 
 }
 
 // index_probe with a top level of the index in LDS: top[j - tj0] = idx[kTopIdx j] (= B[kTopStride j])
-// for every j whose kTopIdx*j lies in the probe's entry range.  The LDS search narrows the index
-// search to kTopIdx entries, so a probe costs log2(kTopIdx) dependent index reads (L2) instead of
-// log2 of the whole range, then one 128-byte line of B as in index_probe.
+// for every j whose kTopIdx*j lies in the probe's entry range, and a middle level in global memory,
+// mid[m] = idx[kMidIdx m].  The LDS search leaves kTopIdx index entries; the middle level's 16
+// entries over them, then the 16 index entries they leave, each load in one round trip; then one
+// 128-byte line of B, as in index_probe.  Three dependent global round trips per probe.
+// Entries <= x among arr[lo, hi) of a sorted array of n entries, hi - lo <= 16: the aligned 16-byte
+// chunks that cover the range load together (one round trip).
+__device__ __forceinline__ uint32_t count_le(const uint32_t *__restrict__ arr, uint64_t n, uint64_t lo, uint64_t hi,
+                                             uint32_t x)
+{
+    uint32_t cnt = 0;
+    const uint64_t c0 = lo & ~3ull;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const uint64_t e = c0 + 4ull * (uint64_t)c;
+        if (e < hi) {
+            const uint4 v = load_chunk(arr, n, e);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t q = e + (uint64_t)j;
+                cnt += (q >= lo && q < hi && w[j] <= x) ? 1u : 0u;
+            }
+        }
+    }
+    return cnt;
+}
+
 __device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__ B, uint64_t pool,
-                                                    const uint32_t *__restrict__ idx, const uint32_t *top,
+                                                    const uint32_t *__restrict__ idx,
+                                                    const uint32_t *__restrict__ mid, const uint32_t *top,
                                                     uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x)
 {
     if (g0 >= g1) return kNoPos;
@@ -128,14 +153,17 @@ __device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__
         if (top[mid - tj0] <= x) lo = mid + 1; else hi = mid;
     }
     // the first index entry > x lies in [L, H]: idx[kTopIdx (j-1)] <= x < idx[kTopIdx j] for j = lo
-    uint64_t L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
-    uint64_t H = (lo < je) ? kTopIdx * lo : ke;
-    while (L < H) {
-        const uint64_t mid = (L + H) >> 1;
-        if (idx[mid] <= x) L = mid + 1; else H = mid;
-    }
-    if (L == 0) return kNoPos;  // x < B[0]
-    const uint64_t base = 32 * (L - 1);  // the line [base, base+32) holds every candidate
+    const uint64_t L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
+    const uint64_t H = (lo < je) ? kTopIdx * lo : ke;
+    // middle level mid[m] = idx[kMidIdx m]: the at most kTopIdx / kMidIdx entries with kMidIdx m in
+    // [L, H) load together and are counted; then the at most kMidIdx index entries they leave
+    const uint64_t mb = (L + kMidIdx - 1) / kMidIdx, me = (H + kMidIdx - 1) / kMidIdx;
+    const uint64_t ms = mb + count_le(mid, (pool + kMidStride - 1) / kMidStride, mb, me, x);
+    const uint64_t L2 = (ms > mb) ? kMidIdx * (ms - 1) + 1 : L;
+    const uint64_t H2 = (ms < me) ? kMidIdx * ms : H;
+    const uint64_t k = L2 + count_le(idx, (pool + 31) / 32, L2, H2, x);  // first index entry > x
+    if (k == 0) return kNoPos;  // x < B[0]
+    const uint64_t base = 32 * (k - 1);  // the line [base, base+32) holds every candidate
     uint64_t hit = kNoPos;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -388,7 +416,7 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     // then compare one 128-byte line of B
                     const uint64_t g = sm.seg_lbase[s];
                     const uint64_t q = top_ok
-                        ? index_probe_top(in.B, in.b_pool, in.b_idx, sm.top, tj0, g + cur, g + hi_s, x)
+                        ? index_probe_top(in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g + cur, g + hi_s, x)
                         : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x);
                     found = q != kNoPos;
                     pos = found ? (uint32_t)(q - g) : cur;

@@ -173,6 +173,7 @@ struct setops_hip_batch {
     uint32_t *stage = nullptr;  // result staging (separate allocation, see stage_reserve)
     uint32_t *b_idx = nullptr;  // index of the B pool (setops_hip_batch_index_b): every 32nd element
     uint32_t *b_top = nullptr;  // its top level: every kTopStride-th element
+    uint32_t *b_mid = nullptr;  // its middle level: every kMidStride-th element
     uint64_t stage_tiles = 0;
     unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
     hipStream_t last_stream = nullptr;
@@ -616,6 +617,7 @@ void batch_free(setops_hip_batch *h)
     if (h->stage) (void)hipFree(h->stage);
     if (h->b_idx) (void)hipFree(h->b_idx);
     if (h->b_top) (void)hipFree(h->b_top);
+    if (h->b_mid) (void)hipFree(h->b_mid);
     delete h;
 }
 
@@ -969,18 +971,21 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
     if (merge_like(handle->kind)) return SETOPS_OK;  // only the search policy reads an index
     const uint64_t m = (handle->in.b_pool + 31) / 32;
     const uint64_t mt = (handle->in.b_pool + kTopStride - 1) / kTopStride;
+    const uint64_t mm = (handle->in.b_pool + kMidStride - 1) / kMidStride;
     if (m == 0) return SETOPS_OK;
     if (hipMalloc(&handle->b_idx, m * 4) != hipSuccess) {
         (void)hipGetLastError();
         handle->b_idx = nullptr;
         return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)(m * 4));
     }
-    if (hipMalloc(&handle->b_top, mt * 4) != hipSuccess) {
+    if (hipMalloc(&handle->b_top, mt * 4) != hipSuccess || hipMalloc(&handle->b_mid, mm * 4) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(handle->b_idx);
+        if (handle->b_top) (void)hipFree(handle->b_top);
         handle->b_idx = nullptr;
         handle->b_top = nullptr;
-        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)(mt * 4));
+        handle->b_mid = nullptr;
+        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)((mt + mm) * 4));
     }
     hipStream_t s = handle->last_stream ? handle->last_stream : g_ctx.stream;
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
@@ -989,9 +994,13 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(mt, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
                        handle->b_top, kTopStride);
     HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(mm, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
+                       handle->b_mid, kMidStride);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     handle->in.b_idx = handle->b_idx;
     handle->in.b_top = handle->b_top;
+    handle->in.b_mid = handle->b_mid;
     return SETOPS_OK;
 }
 

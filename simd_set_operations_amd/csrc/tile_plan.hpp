@@ -43,6 +43,7 @@ struct PairBatch {
     uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
     const uint32_t *b_idx;  // optional: b_idx[k] = B[32k] (search policy: resident B pools)
     const uint32_t *b_top;  // optional, with b_idx: b_top[j] = B[kTopStride j] (staged in LDS per tile)
+    const uint32_t *b_mid;  // with b_top: b_mid[m] = B[kMidStride m]
     // optional, one pair, search policy, plain sets: nx further sets (device pointers and
     // lengths); a probe's result needs it in every one of them as well
     const uint32_t *const *xsets;
@@ -54,6 +55,9 @@ constexpr int kMaxExtraSets = 8;  // further sets per launch (k-set queries run 
 constexpr uint64_t kTopIdx = 256;              // index entries per top-level entry
 constexpr uint64_t kTopStride = 32 * kTopIdx;  // pool elements per top-level entry (8,192)
 constexpr int kTopCap = 2048;                  // top-level entries a search tile stages (8 KB of LDS)
+constexpr uint64_t kMidIdx = 16;               // index entries per middle-level entry
+constexpr uint64_t kMidStride = 32 * kMidIdx;  // pool elements per middle-level entry (512)
+static_assert(kTopIdx / kMidIdx <= 16 && kMidIdx <= 16, "a level spans at most 16 entries of the next");
 
 // Index of a resident pool: idx[k] = pool[stride * k], k < ceil(n / stride).
 __global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx,
