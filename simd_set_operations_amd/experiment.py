@@ -18,8 +18,8 @@ Mirrors the reference's benchmark crate:
   reference's Results JSON shape (x, times in ns; the CPU perf-counter fields are null).
 
 Algorithm names resolve to GPU entries: a registered name is used as is, otherwise ``hip_`` + the
-name, with the reference's SIMD-width suffixes dropped (``shuffling_avx512`` -> ``hip_shuffling``:
-the vector width is a CPU notion).  A name with no GPU entry is reported and skipped, as the
+name, with the reference's SIMD-width and ``_branch`` suffixes dropped (``shuffling_avx512_branch``
+-> ``hip_shuffling``: vector width and branching are CPU notions).  A name with no GPU entry is reported and skipped, as the
 reference skips an unknown algorithm.  Generation uses a seeded PCG64 stream per datafile instead
 of the reference's thread-local RNG, so datasets are reproducible.
 """
@@ -268,6 +268,8 @@ def gpu_entry_name(name, registered):
     if name in registered:
         return name
     base = name
+    if base.endswith("_branch"):  # branch vs branch-free is a CPU notion too
+        base = base[:-len("_branch")]
     bsr = base.endswith("_bsr")
     if bsr:
         base = base[:-4]

@@ -187,4 +187,6 @@ def test_gpu_entry_names():
     assert ex.gpu_entry_name("shuffling_avx512", reg) == "hip_shuffling"
     assert ex.gpu_entry_name("shuffling_sse_bsr", reg) == "hip_shuffling_bsr"
     assert ex.gpu_entry_name("hip_galloping", reg) == "hip_galloping"
+    assert ex.gpu_entry_name("shuffling_avx2_bsr_branch", reg) == "hip_shuffling_bsr"
+    assert ex.gpu_entry_name("shuffling_sse_branch", reg) == "hip_shuffling"
     assert ex.gpu_entry_name("croaring", reg) is None
