@@ -263,19 +263,20 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     e.close()
     res["merge_bsr"] = {"ms": ms, "sum": s3b, "gelems": E / ms / 1e6}
     best = None
-    for thr in (32.0, 128.0, 512.0):
+    for thr, indexed in ((32.0, False), (128.0, False), (512.0, False), (32.0, True), (128.0, True)):
         sk = split_by_skew(deg[u], deg[v], thr)
-        a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk)
+        a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk, index=indexed)
         b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
         ms, sh, _ = timed([a, b])
         a.close()
         b.close()
-        res[f"split_{int(thr)}"] = {"ms": ms, "sum": sh, "search_edges": int(sk.sum()), "gelems": E / ms / 1e6}
+        key = f"split_{int(thr)}" + ("_indexed" if indexed else "")
+        res[key] = {"ms": ms, "sum": sh, "search_edges": int(sk.sum()), "gelems": E / ms / 1e6}
         if sh != s3:
-            log(f"graph split {thr}: sum {sh} differs from {s3}")
+            log(f"graph {key}: sum {sh} differs from {s3}")
             sys.exit(3)
         if best is None or ms < res[best]["ms"]:
-            best = f"split_{int(thr)}"
+            best = key
     res["best_split"] = best
     if s3 != 3 * t or s3b != s3:
         log(f"graph: sums disagree (merge {s3}, bsr {s3b}, oriented {t})")

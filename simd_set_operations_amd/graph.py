@@ -46,13 +46,15 @@ class EdgeIntersections:
     """Count-only gather batches over the edges (u, v), u < v, of a symmetric CSR already on the
     device (col / bases / states as device tensors)."""
 
-    def __init__(self, algo, pool, beg, end, u, v, states=None, select=None):
+    def __init__(self, algo, pool, beg, end, u, v, states=None, select=None, index=False):
         uu, vv = (u, v) if select is None else (u[select], v[select])
         self.pairs = int(uu.size)
         self.batch = None
         if self.pairs:
             self.batch = GatherBatch(algo, pool, beg[uu], end[uu], pool, beg[vv], end[vv], A_states=states,
                                      B_states=states, outputs=False)
+            if index:
+                self.batch.index_b()  # search-policy entries: the CSR pool indexed once
 
     def run(self, stream=None):
         if self.batch is not None:
