@@ -470,6 +470,10 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             }
             plan.tile_count[t] = tot;
             plan.tile_elems[t] = el;
+            if (plan.direct && out.totals) {  // one tile: its counts are the totals
+                out.totals[0] = tot;
+                out.totals[1] = el;
+            }
         }
         if (tid < nseg) {
             // results of the tile's units before x: whole lanes, then the lane's results before x

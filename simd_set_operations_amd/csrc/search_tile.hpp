@@ -489,6 +489,10 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             }
             plan.tile_count[t] = tot;
             plan.tile_elems[t] = el;
+            if (plan.direct && out.totals) {  // one tile: its counts are the totals
+                out.totals[0] = tot;
+                out.totals[1] = el;
+            }
         }
         if (tid < nseg) {
             auto before = [&](uint32_t x) -> uint32_t {

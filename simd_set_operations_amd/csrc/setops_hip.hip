@@ -413,8 +413,11 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
         int rc = launch_units_scan(h, out, stream);
         if (rc != SETOPS_OK) return rc;
     }
-    hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in, h->plan);
-    HIP_TRY(hipGetLastError());
+    if (!h->plan.direct) {
+        hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in,
+                           h->plan);
+        HIP_TRY(hipGetLastError());
+    }
     // count (and stage)
     if (ev) HIP_TRY(hipEventRecord(ev->first, stream));
     if (h->bsr) {
@@ -426,6 +429,7 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     }
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev->second, stream));
+    if (h->plan.direct) return SETOPS_OK;  // results and totals are final
     // scan: with few groups one kernel (a workgroup adds the earlier groups' tile counts itself);
     // otherwise group sums, one-block scan of the group sums (writes the totals), per-group scan
     const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
@@ -486,6 +490,7 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     if (!count_only) {
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;
+        if (h->plan.direct) h->plan.stage = d_out;  // the one tile's slot is the output
     }
     h->last_stream = stream;
     if (h->timing || !h->use_graph) {
@@ -580,6 +585,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->in.P = npairs;
     h->in.contig = gather ? 0u : 1u;
     h->plan.single = 0;
+    h->plan.direct = 0;
     int rc;
     if (!gather && merge_like(e->kind)) {
         rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)kC0 * npairs);
@@ -680,6 +686,8 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     h->in.nx = nx;
     h->plan.single_na = na;
     h->plan.single_nb = nb;
+    // one tile, plain sets, and room for any result: the tile writes the output and the totals
+    h->plan.direct = (h->plan.ntiles == 1 && !e->bsr && dTot && (dO == nullptr || cap >= std::min(na, nb))) ? 1u : 0u;
     return batch_launch(h, dO, dOs, cap, nullptr, nullptr, dTot, s);
 }
 

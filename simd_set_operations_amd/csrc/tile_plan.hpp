@@ -96,6 +96,9 @@ struct TilePlan {
     uint32_t single;
     uint64_t *single_off;
     uint64_t single_na, single_nb;
+    // single pair in one tile, plain sets, output capacity >= min(na, nb): `stage` is the caller's
+    // output, the tile kernel writes the totals, and the split, scan and placement are skipped
+    uint32_t direct;
 };
 
 struct TileOut {
@@ -152,6 +155,15 @@ __global__ void plan_csr_kernel(PairBatch in, TilePlan plan, TileOut out)
         plan.single_off[1] = plan.single_na;
         plan.single_off[2] = 0;
         plan.single_off[3] = plan.single_nb;
+        if (plan.direct) {
+            // the one tile starts at the pair's start (the split of unit 0); its runs are the sets
+            plan.tile_sa[0] = 0;
+            plan.tile_sb[0] = 0;
+            plan.tile_ga[0] = 0;
+            plan.tile_ga[1] = plan.single_na;
+            plan.tile_gb[0] = 0;
+            plan.tile_gb[1] = plan.single_nb;
+        }
     }
     const uint64_t a0 = abeg(0), b0 = bbeg(0);
     for (uint64_t p = gid; p < in.P; p += gsz) {
