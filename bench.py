@@ -300,6 +300,47 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     return res
 
 
+def bsr_leg(torch, A, a_off, B, b_off, out_total, steps, warmup):
+    """The uneven batch in the BSR form (base = v >> 5, 32-bit state; setops/src/bsr.rs:140-164),
+    encoded outside the timed region as the reference's harness does (harness.rs:121-122), through
+    hip_branchless_merge_bsr with outputs.  Throughput counts the original elements; the result's
+    popcount total must equal the plain result count."""
+    from simd_set_operations_amd.batch import DeviceBatch
+    from simd_set_operations_amd.graph import csr_to_bsr
+
+    a_bo, a_bases, a_states = csr_to_bsr(a_off, A)
+    b_bo, b_bases, b_states = csr_to_bsr(b_off, B)
+    E = int(A.size + B.size)
+    db = DeviceBatch("hip_branchless_merge_bsr", a_bases, a_bo, b_bases, b_bo, A_states=a_states, B_states=b_states)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        db.run(stream)
+    torch.cuda.synchronize()
+    words, elems = db.totals()
+    if elems != out_total:
+        log(f"bsr: popcount total {elems} differs from {out_total}")
+        sys.exit(3)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        db.run(stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    db.kernel_time()
+    db.set_timing(True)
+    for _ in range(steps):
+        db.run(stream)
+    torch.cuda.synchronize()
+    k_ms, runs = db.kernel_time()
+    db.close()
+    r = {"words_in": int(a_bases.size + b_bases.size), "elements_in": E, "result_words": words,
+         "result_elements": elems, "ms_per_step": ms, "kernel_ms": k_ms / max(runs, 1), "gelems": E / ms / 1e6}
+    log(f"bsr: {r}")
+    return r
+
+
 def kset_leg(with_cpu):
     """k-set queries through the host-pointer k-set call (the plain drop-in for svs_generic and
     small_adaptive): sets generated as the reference's gen_kset (generators.rs:113-135) with
@@ -402,6 +443,7 @@ def main():
     ap.add_argument("--graph-scale", type=int, default=22)
     ap.add_argument("--no-serving", action="store_true", help="skip the served k-set leg")
     ap.add_argument("--no-kset", action="store_true", help="skip the host-pointer k-set leg")
+    ap.add_argument("--no-bsr", action="store_true", help="skip the BSR form of the uneven batch")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
     ap.add_argument("--corpus-mode", choices=("iid", "rank"), default="iid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -528,6 +570,8 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_bsr:
+        extra["bsr"] = bsr_leg(torch, A, a_off, B, b_off, out_total, args.steps, args.warmup)
     if not args.no_kset:
         extra["kset"] = kset_leg(not args.no_baselines)
     if not args.no_serving:

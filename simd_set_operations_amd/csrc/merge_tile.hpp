@@ -152,8 +152,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         // the lane moved to the next pair at step kt, from that pair's window start (pa1).
         const uint32_t ub = tid * K;
         uint32_t smask = 0, tmask = 0, pa0 = 0, pa1 = 0;
-        uint32_t svals[(BSR && !BLOCKP) ? K : 1];
         uint32_t nelems = 0;
+        uint32_t shift_o = 0, pbl = 0;  // merge policy: masks' shift, loop's first B index
         uint32_t kt = K;  // merge policy: step at which the lane enters the next pair
         uint32_t s_first = 0;  // plain merge: the lane's first segment
         bool lane_on = false;  // plain merge: the lane has units in the tile
@@ -255,7 +255,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 pb -= x;
             }
         }
-        if constexpr (!BSR) {
+        {
             // One branch-free loop of K merge steps for every wave; a step re-reads both current
             // values (two LDS reads, no selects).  What a checked merge step
             // would test each time is settled per lane before and after the loop:
@@ -281,6 +281,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             const uint32_t kv = kmax - shift;
             pa0 = pa;
             pa1 = pa;
+            pbl = pb;
+            shift_o = shift;
             uint32_t a = sm.key[pa], b = sm.key[pb];
             // oa: byte offset of A's current element; B's is s4 + 4*steps - oa.  The empty asm
             // keeps oa the one position variable (otherwise the compiler rebuilds both offsets
@@ -312,82 +314,22 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 }
                 smask = keep;
             }
+            if constexpr (BSR) {
+                // a base match emits (base, state_a & state_b) only when the AND is non-zero
+                // (merge.rs:62-87): the A element taken at step k and the B element current then
+                uint32_t m = smask, keep = 0;
+                while (m) {
+                    const uint32_t k = __builtin_ctz(m);
+                    m &= m - 1;
+                    const uint32_t ta = __popc(tmask & ((1u << k) - 1u));
+                    const uint32_t stv = sm.st[pa + ta] & sm.st[pb + k - ta];
+                    keep |= stv ? (1u << k) : 0u;
+                    nelems += __popc(stv);
+                }
+                smask = keep;
+            }
             smask <<= shift;
             tmask <<= shift;
-        } else {
-            pa0 = pa;
-            uint32_t a = sm.key[pa], b = sm.key[pb];
-            // One merge step, branch-free: consume A when it is not exhausted and not larger than the
-            // current B (or B is exhausted); emit when the consumed A equals the current B.  One LDS
-            // read per step (the side that advanced).
-            auto step = [&](uint32_t k) {
-                const bool aok = pa < ea, bok = pb < eb;
-                const bool takeA = aok & ((!bok) | (a <= b));
-                bool hit = takeA & (pb < ebp) & (a == b);
-                if constexpr (BSR) {
-                    uint32_t stv = 0;
-                    if (hit) stv = sm.st[pa] & sm.st[pb];
-                    hit = hit && stv != 0;
-                    svals[k] = stv;
-                    nelems += __popc(stv);
-                }
-                smask |= hit ? (1u << k) : 0u;
-                tmask |= takeA ? (1u << k) : 0u;
-                pa += takeA;
-                pb += (!takeA) & bok;
-                const uint32_t v = sm.key[takeA ? pa : pb];
-                a = takeA ? v : a;
-                b = takeA ? b : v;
-            };
-            // Fast path: K merge steps stay inside both windows for every lane of the wave (no early
-            // end, no pair change, >= K elements left on each side), so neither side can run out and
-            // the boundary element cannot be reached: take A iff a <= b, and a == b is a hit.
-            auto step_fast = [&](uint32_t k) {
-                const bool takeA = a <= b;
-                bool hit = a == b;
-                if constexpr (BSR) {
-                    uint32_t stv = 0;
-                    if (hit) stv = sm.st[pa] & sm.st[pb];
-                    hit = hit && stv != 0;
-                    svals[k] = stv;
-                    nelems += __popc(stv);
-                }
-                smask |= hit ? (1u << k) : 0u;
-                tmask |= takeA ? (1u << k) : 0u;
-                pa += takeA;
-                pb += !takeA;
-                const uint32_t v = sm.key[takeA ? pa : pb];
-                a = takeA ? v : a;
-                b = takeA ? b : v;
-            };
-            const bool uniform = ballot(kmax != (uint32_t)K || kt != (uint32_t)K) == 0;
-            const bool fast = uniform && ballot(pa + K > ea || pb + K > eb) == 0;
-            if (fast) {
-    #pragma unroll
-                for (int k = 0; k < K; k++) step_fast((uint32_t)k);
-            } else if (uniform) {
-                // no lane of this wave ends its tile early or crosses into another pair
-    #pragma unroll
-                for (int k = 0; k < K; k++) step((uint32_t)k);
-            } else {
-    #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if ((uint32_t)k == kt) {
-                        // next pair: its window starts at the pair's first elements
-                        s++;
-                        pa = sm.seg_la[s];
-                        pb = sm.seg_lb[s];
-                        pa1 = pa;
-                        ea = pa + sm.seg_wa[s];
-                        eb = pb + sm.seg_wb[s];
-                        ebp = pb + sm.seg_nb[s];
-                        a = sm.key[pa];
-                        b = sm.key[pb];
-                    }
-                    if ((uint32_t)k < kmax) step((uint32_t)k);
-                }
-            }
-
         }
         }
         const uint32_t nwords = __popc(smask);
@@ -443,13 +385,17 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                     pos++;
                 }
             } else if constexpr (BSR) {
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if (smask & (1u << k)) {
-                        dk[pos] = sm.key[a_index((uint32_t)k)];
-                        ds[pos] = svals[k];
-                        pos++;
-                    }
+                // the matches' bases and state ANDs, re-read from LDS (A at the emit step, B the
+                // element current then)
+                uint32_t sm_ = smask;
+                while (sm_) {
+                    const uint32_t k = __builtin_ctz(sm_);
+                    sm_ &= sm_ - 1;
+                    const uint32_t ta = __popc(tmask & ((1u << k) - 1u));
+                    const uint32_t ai = pa0 + ta, bi = pbl + (k - shift_o) - ta;
+                    dk[pos] = sm.key[ai];
+                    ds[pos] = sm.st[ai] & sm.st[bi];
+                    pos++;
                 }
             } else {
                 uint32_t sm_ = smask;
