@@ -31,9 +31,9 @@ struct MergeSmem {
     alignas(16) uint32_t key[KEYS + K + 8];
     alignas(16) uint32_t st[BSR ? KEYS : 4];
     static_assert(KEYS + K + 8 < 65536 && U < 65536, "segment fields are 16-bit LDS indices");
-    // plain merge: per-segment result prefixes replace the per-lane masks (LDS is what limits how
+    // merge policy: per-segment result prefixes replace the per-lane masks (LDS is what limits how
     // many workgroups share a CU, so the segment table is 16-bit)
-    static constexpr bool LEAN = !BSR && !BLOCKP;
+    static constexpr bool LEAN = !BLOCKP;
     uint16_t seg_u[MAXSEG + 1];  // tile-relative unit where segment s starts; [nseg] = tile length
     uint16_t seg_la[MAXSEG];     // LDS index of the segment's first A element in the window
     uint16_t seg_lb[MAXSEG];     // LDS index of the segment's first B element in the window
