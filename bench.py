@@ -30,8 +30,9 @@ DOMINANT_KERNEL = {
     "hip_branchless_merge": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 0>",
     "hip_shuffling": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 1>",
     "hip_qfilter": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 2>",
-    "hip_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, true>",
-    "hip_binary_search": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, false>",
+    "hip_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 1>",
+    "hip_binary_search": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 0>",
+    "hip_simd_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 2>",
 }
 
 
@@ -162,8 +163,8 @@ def skewed_leg(torch, steps, cpu_threads, with_cpu):
            "elements_streamed": int(P * (L.size + m)), "bytes_unique": B_unique}
     dL = torch.from_numpy(L.view(np.int32)).cuda()
     stream = torch.cuda.current_stream()
-    for algo, indexed in (("hip_galloping", False), ("hip_binary_search", False), ("hip_galloping", True),
-                          ("hip_binary_search", True)):
+    for algo, indexed in (("hip_galloping", False), ("hip_binary_search", False), ("hip_simd_galloping", False),
+                          ("hip_galloping", True), ("hip_binary_search", True), ("hip_simd_galloping", True)):
         gb = GatherBatch(algo, pool, a_beg, a_end, dL, b_beg, b_end)
         key = algo + ("_indexed" if indexed else "")
         if indexed:

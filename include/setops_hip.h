@@ -134,8 +134,8 @@ int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const ui
                                    uint64_t b_len, const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs);
 
 /* Builds an index of the batch's B pool (every 32nd element, 1/32 of the pool, kept until the
- * handle is destroyed) and lets the search-policy entries (hip_galloping, hip_binary_search and
- * their BSR twins) use it for larger sets that do not fit on chip: a probe then reads about one
+ * handle is destroyed) and lets the search-policy entries (hip_galloping, hip_binary_search,
+ * hip_simd_galloping and their BSR twins) use it for larger sets that do not fit on chip: a probe then reads about one
  * 128-byte line of B instead of one per binary-search level.  For resident sets that many pairs
  * share (a large set against many small ones, a corpus).  The B pool's contents must not change
  * afterwards where pairs read them.  Waits for the index build.  No-op for merge-policy entries.

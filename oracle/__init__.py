@@ -10,12 +10,14 @@ only as the checker (or the timed CPU baseline) -- never as the thing measured o
   build time).
 
 Pinning (what checks each oracle function, see tests/test_oracle.py):
-  branchless_merge / naive_merge / galloping / binary_search_intersect / shuffling / qfilter:
+  branchless_merge / naive_merge / galloping / binary_search_intersect / shuffling / qfilter /
+  broadcast / simd_galloping:
       the reference's known answers (setops/tests/unit_tests.rs:6-59), numpy.intersect1d (a
       library routine), brute-force membership on tiny universes, the invariant trio of
       setops/tests/testlib/properties.rs:2-49, the compiled reference qfilter (_ref), and the
       committed golden vectors written from _ref.
-  branchless_merge_bsr / galloping_bsr / shuffling_bsr / qfilter_bsr:
+  branchless_merge_bsr / galloping_bsr / shuffling_bsr / qfilter_bsr / broadcast_bsr /
+  simd_galloping_bsr:
       decode(result) == plain intersection, the compiled reference intersect_qfilter_bsr_b4_v2.
   bsr_from_sorted / bsr_to_sorted: round trip, popcount == length, bases strictly increasing.
   svs: left fold of numpy.intersect1d == k-way intersection (reduce over all sets).
@@ -37,6 +39,9 @@ SHUFFLING4 = 4
 SHUFFLING8 = 5
 SHUFFLING16 = 6
 QFILTER = 7
+BROADCAST4 = 8
+SIMD_GALLOPING4 = 9
+SIMD_GALLOPING16 = 10
 
 ALGOS = {
     "naive_merge": NAIVE_MERGE,
@@ -47,6 +52,9 @@ ALGOS = {
     "shuffling_avx2": SHUFFLING8,
     "shuffling_avx512": SHUFFLING16,
     "qfilter": QFILTER,
+    "broadcast_sse": BROADCAST4,
+    "galloping_sse": SIMD_GALLOPING4,
+    "galloping_avx512": SIMD_GALLOPING16,
 }
 
 _c = ctypes
@@ -76,8 +84,14 @@ def lib():
             f = getattr(L, "oracle_" + n)
             f.restype = _sz
             f.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
-        L.oracle_shuffling_bsr.restype = _sz
-        L.oracle_shuffling_bsr.argtypes = [_c.c_uint, _vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
+        for n in ("oracle_shuffling_bsr", "oracle_broadcast_bsr", "oracle_simd_galloping_bsr"):
+            f = getattr(L, n)
+            f.restype = _sz
+            f.argtypes = [_c.c_uint, _vp, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
+        for n in ("oracle_broadcast", "oracle_simd_galloping"):
+            f = getattr(L, n)
+            f.restype = _sz
+            f.argtypes = [_c.c_uint, _vp, _sz, _vp, _sz, _vp]
         L.oracle_bsr_from_sorted.restype = _sz
         L.oracle_bsr_from_sorted.argtypes = [_vp, _sz, _vp, _vp]
         L.oracle_bsr_to_sorted.restype = _sz
@@ -167,14 +181,15 @@ def bsr_to_sorted(bases, states):
 
 
 def bsr_intersect(kind, ba, sa, bb, sb, width=4):
-    """kind in {'branchless_merge_bsr', 'galloping_bsr', 'shuffling_bsr', 'qfilter_bsr'}."""
+    """kind in {'branchless_merge_bsr', 'galloping_bsr', 'shuffling_bsr', 'qfilter_bsr', 'broadcast_bsr',
+    'simd_galloping_bsr'}; width is W for the shuffling, broadcast and simd_galloping forms."""
     ba, sa, bb, sb = _u32(ba), _u32(sa), _u32(bb), _u32(sb)
     cap = max(min(ba.size, bb.size), 1)
     ob = np.zeros(cap, np.uint32)
     os_ = np.zeros(cap, np.uint32)
     L = lib()
-    if kind == "shuffling_bsr":
-        n = L.oracle_shuffling_bsr(width, _p(ba), _p(sa), ba.size, _p(bb), _p(sb), bb.size, _p(ob), _p(os_))
+    if kind in ("shuffling_bsr", "broadcast_bsr", "simd_galloping_bsr"):
+        n = getattr(L, "oracle_" + kind)(width, _p(ba), _p(sa), ba.size, _p(bb), _p(sb), bb.size, _p(ob), _p(os_))
     else:
         n = getattr(L, "oracle_" + kind)(_p(ba), _p(sa), ba.size, _p(bb), _p(sb), bb.size, _p(ob), _p(os_))
     return ob[:n].copy(), os_[:n].copy()

@@ -17,6 +17,10 @@
  *   oracle_galloping_bsr          setops/src/intersect/galloping.rs:51-76
  *   oracle_shuffling              setops/src/intersect/shuffling.rs:26-168 (W = 4, 8, 16)
  *   oracle_shuffling_bsr          setops/src/intersect/shuffling.rs:174-356
+ *   oracle_broadcast              setops/src/intersect/broadcast.rs:21-160 (W = 4, 8, 16)
+ *   oracle_broadcast_bsr          setops/src/intersect/broadcast.rs:163-320
+ *   oracle_simd_galloping         setops/src/intersect/simd_galloping.rs:19-107, 196-296
+ *   oracle_simd_galloping_bsr     setops/src/intersect/simd_galloping.rs:109-193
  *   oracle_qfilter                setops/src/intersect/qfilter.rs:32-89 (+ dictionary :254-357)
  *   oracle_qfilter_bsr            setops/src/intersect/qfilter.rs:92-180
  *   oracle_bsr_from_sorted        setops/src/bsr.rs:140-164
@@ -249,6 +253,173 @@ size_t oracle_shuffling_bsr(unsigned W, const uint32_t *ba, const uint32_t *sa, 
     return s.words + tail;
 }
 
+/* broadcast.rs:21-160: the shuffling block walk, with each B lane of the block splatted and
+ * compared against the whole A block (one compare vector per B lane) instead of rotations. */
+size_t oracle_broadcast(unsigned W, const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out)
+{
+    size_t st_a = (na / W) * W, st_b = (nb / W) * W;
+    size_t i = 0, j = 0, count = 0;
+    while (i < st_a && j < st_b) {
+        unsigned mask = 0;
+        for (unsigned y = 0; y < W; y++)
+            for (unsigned x = 0; x < W; x++) mask |= (unsigned)(a[i + x] == b[j + y]) << x;
+        for (unsigned x = 0; x < W; x++)
+            if (mask >> x & 1u) EMIT(a[i + x]);
+        uint32_t amax = a[i + W - 1], bmax = b[j + W - 1];
+        i += W * (size_t)(amax <= bmax);
+        j += W * (size_t)(bmax <= amax);
+    }
+    size_t tail = oracle_branchless_merge(a + i, na - i, b + j, nb - j, out ? out + count : NULL);
+    return count + tail;
+}
+
+/* broadcast.rs:163-320: per splatted B lane, (base equal) & state_a & state_b, OR-ed over the B
+ * lanes; an A lane is emitted when some base matched and the OR is non-zero. */
+size_t oracle_broadcast_bsr(unsigned W, const uint32_t *ba, const uint32_t *sa, size_t na,
+                            const uint32_t *bb, const uint32_t *sb, size_t nb,
+                            uint32_t *out_bases, uint32_t *out_states)
+{
+    bsr_sink s = {out_bases, out_states, 0, 0};
+    size_t st_a = (na / W) * W, st_b = (nb / W) * W;
+    size_t i = 0, j = 0;
+    uint32_t st[16];
+    while (i < st_a && j < st_b) {
+        unsigned base_mask = 0;
+        for (unsigned x = 0; x < W; x++) st[x] = 0;
+        for (unsigned y = 0; y < W; y++)
+            for (unsigned x = 0; x < W; x++) {
+                int eq = ba[i + x] == bb[j + y];
+                base_mask |= (unsigned)eq << x;
+                st[x] |= eq ? (sa[i + x] & sb[j + y]) : 0u;
+            }
+        for (unsigned x = 0; x < W; x++)
+            if ((base_mask >> x & 1u) && st[x] != 0) bsr_emit(&s, ba[i + x], st[x]);
+        uint32_t amax = ba[i + W - 1], bmax = bb[j + W - 1];
+        i += W * (size_t)(amax <= bmax);
+        j += W * (size_t)(bmax <= amax);
+    }
+    size_t tail = oracle_branchless_merge_bsr(ba + i, sa + i, na - i, bb + j, sb + j, nb - j,
+                                              out_bases ? out_bases + s.words : NULL,
+                                              out_states ? out_states + s.words : NULL);
+    return s.words + tail;
+}
+
+/* simd_galloping.rs:196-243 (gallop_wide + binary_search_wide): blocks of `bound` elements of
+ * large[0, nl), nl >= bound; the first full block whose last element is >= target, found by
+ * doubling a block offset while the block after it ends below the target and binary-searching
+ * [offset/2, min(offset, nl/bound - 1)].  Returns the last full block when none qualifies. */
+static size_t sg_gallop_wide(uint32_t target, const uint32_t *large, size_t nl, size_t bound)
+{
+    size_t upper = 0;
+    if (large[bound - 1] < target) {
+        upper = 1;
+        while ((upper + 1) * bound - 1 < nl && large[(upper + 1) * bound - 1] < target) upper *= 2;
+    }
+    size_t lo = upper / 2, hi = nl / bound - 1;
+    if (upper < hi) hi = upper;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (large[(mid + 1) * bound - 1] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+/* simd_galloping.rs:49-107: Lemire's SIMD galloping with bound = 32 W.  The smaller set probes;
+ * each target gallops over whole bound-sized blocks of the larger set; when no block can hold
+ * it the larger set drops its full blocks and, if the probe side is at least a block long, the
+ * two sides swap; else the block is narrowed to the quarter that can hold the target
+ * (reduce_search_bound, :245-264) and its 8 W elements are compared (block_compare, :266-296).
+ * Whatever is left (the larger side shorter than a block) goes through branchless_merge. */
+size_t oracle_simd_galloping(unsigned W, const uint32_t *small, size_t ns, const uint32_t *large, size_t nl,
+                             uint32_t *out)
+{
+    size_t count = 0;
+    const size_t bound = 32 * (size_t)W;
+    if (ns > nl) {
+        const uint32_t *t = small; small = large; large = t;
+        size_t tn = ns; ns = nl; nl = tn;
+    }
+    while (ns > 0 && nl >= bound) {
+        uint32_t target = small[0];
+        size_t blk = sg_gallop_wide(target, large, nl, bound);
+        if (large[(blk + 1) * bound - 1] < target) {
+            large += (blk + 1) * bound;
+            nl -= (blk + 1) * bound;
+            if (ns >= bound) {
+                const uint32_t *t = small; small = large; large = t;
+                size_t tn = ns; ns = nl; nl = tn;
+                continue;
+            }
+            break;
+        }
+        large += blk * bound;
+        nl -= blk * bound;
+        size_t q;  /* first register (of W elements) of the quarter that can hold the target */
+        if (large[bound / 2 - 1] >= target)
+            q = (large[bound / 4 - 1] < target) ? 8 : 0;
+        else
+            q = (large[bound * 3 / 4 - 1] < target) ? 24 : 16;
+        int hit = 0;
+        for (size_t e = W * q; e < W * (q + 8); e++) hit |= large[e] == target;
+        if (hit) EMIT(target);
+        small++;
+        ns--;
+    }
+    size_t tail = oracle_branchless_merge(small, ns, large, nl, out ? out + count : NULL);
+    return count + tail;
+}
+
+/* simd_galloping.rs:109-193: the same walk on bases with bound = W (one register per block): the
+ * first matching base of the found block gives the state, target_state & state, emitted non-zero;
+ * branchless_merge_bsr for the rest. */
+size_t oracle_simd_galloping_bsr(unsigned W, const uint32_t *ba, const uint32_t *sa, size_t ns,
+                                 const uint32_t *bb, const uint32_t *sb, size_t nl,
+                                 uint32_t *out_bases, uint32_t *out_states)
+{
+    bsr_sink s = {out_bases, out_states, 0, 0};
+    const size_t bound = W;
+    if (ns > nl) {
+        const uint32_t *t = ba; ba = bb; bb = t;
+        t = sa; sa = sb; sb = t;
+        size_t tn = ns; ns = nl; nl = tn;
+    }
+    while (ns > 0 && nl >= bound) {
+        uint32_t target = ba[0];
+        size_t blk = sg_gallop_wide(target, bb, nl, bound);
+        if (bb[(blk + 1) * bound - 1] < target) {
+            bb += (blk + 1) * bound;
+            sb += (blk + 1) * bound;
+            nl -= (blk + 1) * bound;
+            if (ns >= bound) {
+                const uint32_t *t = ba; ba = bb; bb = t;
+                t = sa; sa = sb; sb = t;
+                size_t tn = ns; ns = nl; nl = tn;
+                continue;
+            }
+            break;
+        }
+        bb += blk * bound;
+        sb += blk * bound;
+        nl -= blk * bound;
+        for (size_t e = 0; e < bound; e++)
+            if (bb[e] == target) {
+                uint32_t st = sa[0] & sb[e];
+                if (st != 0) bsr_emit(&s, target, st);
+                break;
+            }
+        ba++;
+        sa++;
+        ns--;
+    }
+    size_t tail = oracle_branchless_merge_bsr(ba, sa, ns, bb, sb, nl,
+                                              out_bases ? out_bases + s.words : NULL,
+                                              out_states ? out_states + s.words : NULL);
+    return s.words + tail;
+}
+
 /* qfilter.rs:254-357: classification of the 16-bit low-byte equality mask.  Bit 4*x + y is set
  * when the low byte of a[x] equals the low byte of b[y] (BYTE_CHECK_GROUP_A/B, instructions.rs:100-111).
  * Returns -2 (no lane has a candidate), -1 (some lane has several candidates) or the packed 2-bit
@@ -412,6 +583,9 @@ enum {
     ORACLE_SHUFFLING8 = 5,
     ORACLE_SHUFFLING16 = 6,
     ORACLE_QFILTER = 7,
+    ORACLE_BROADCAST4 = 8,
+    ORACLE_SIMD_GALLOPING4 = 9,
+    ORACLE_SIMD_GALLOPING16 = 10,
 };
 
 size_t oracle_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out)
@@ -425,6 +599,9 @@ size_t oracle_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t 
     case ORACLE_SHUFFLING8: return oracle_shuffling(8, a, na, b, nb, out);
     case ORACLE_SHUFFLING16: return oracle_shuffling(16, a, na, b, nb, out);
     case ORACLE_QFILTER: return oracle_qfilter(a, na, b, nb, out);
+    case ORACLE_BROADCAST4: return oracle_broadcast(4, a, na, b, nb, out);
+    case ORACLE_SIMD_GALLOPING4: return oracle_simd_galloping(4, a, na, b, nb, out);
+    case ORACLE_SIMD_GALLOPING16: return oracle_simd_galloping(16, a, na, b, nb, out);
     default: return (size_t)-1;
     }
 }

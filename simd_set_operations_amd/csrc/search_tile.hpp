@@ -78,6 +78,89 @@ __device__ __forceinline__ uint32_t lower_bound_from(const Arr &arr, uint32_t lo
     return lo;
 }
 
+// Search modes of the probes: binary search, galloping, or galloping over blocks with a full
+// compare of the last block (simd_galloping).
+enum : int { SM_BINARY = 0, SM_GALLOP = 1, SM_BLOCK = 2 };
+constexpr uint32_t kBlockLds = 8;    // block of a staged (LDS) window: 8 reads
+constexpr uint32_t kBlockPool = 32;  // block of a pool window: one 128-byte span, nine 16-byte loads
+
+// Blocks [lo + W b, lo + W b + W) of arr[lo, hi): the first block whose last element is >= x, found
+// by galloping over the blocks' last elements (blocks 0, 1, 2, 4, 8, ...) and a binary search of the
+// blocks the gallop leaves -- gallop_wide and binary_search_wide of simd_galloping.rs:196-243 with
+// GPU-sized blocks.  Returns the block, or the block count when every element is < x.
+template <uint32_t W, class Arr>
+__device__ __forceinline__ uint32_t gallop_blocks(const Arr &arr, uint32_t lo, uint32_t hi, uint32_t x)
+{
+    const uint32_t nbk = (hi - lo + W - 1) / W;
+    auto last = [&](uint32_t b) {
+        const uint32_t e = lo + W * b + W;
+        return arr((e < hi ? e : hi) - 1);
+    };
+    uint32_t bl = 0, bh = 0, step = 1;
+    while (bh < nbk && last(bh) < x) {
+        bl = bh + 1;
+        bh = (step >= nbk) ? nbk : step;
+        step <<= 1;
+    }
+    while (bl < bh) {  // blocks < bl end below x; block bh (if any) ends at >= x
+        const uint32_t mid = (bl + bh) >> 1;
+        if (last(mid) < x) bl = mid + 1; else bh = mid;
+    }
+    return bl;
+}
+
+// Lower bound of x in arr[lo, hi) by block galloping; the block that can hold x is counted in
+// full, W independent reads in place of the last log2(W) dependent steps (block_compare,
+// simd_galloping.rs:272-296).  For LDS arrays.
+template <uint32_t W, class Arr>
+__device__ __forceinline__ uint32_t lower_bound_block(const Arr &arr, uint32_t lo, uint32_t hi, uint32_t x)
+{
+    const uint32_t b = gallop_blocks<W>(arr, lo, hi, x);
+    const uint32_t s0 = lo + W * b;
+    if (s0 >= hi) return hi;
+    const uint32_t n = (hi - s0 < W) ? hi - s0 : W;
+    uint32_t lt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < W; j++) lt += (j < n && arr(s0 + j) < x) ? 1u : 0u;
+    return s0 + lt;
+}
+
+// lower_bound_block over a window of a resident pool, L = P + g, P readable up to pool elements
+// and 16-byte aligned: the last block is read as the aligned 16-byte chunks that cover it (one
+// round trip).  Also reports whether the window holds x.
+__device__ __forceinline__ uint32_t lower_bound_block_pool(const uint32_t *__restrict__ P, uint64_t pool, uint64_t g,
+                                                           uint32_t lo, uint32_t hi, uint32_t x, bool &found)
+{
+    constexpr uint32_t W = kBlockPool;
+    const uint32_t *__restrict__ L = P + g;
+    auto arr = [&](uint32_t i) { return L[i]; };
+    const uint32_t b = gallop_blocks<W>(arr, lo, hi, x);
+    const uint32_t s0 = lo + W * b;
+    found = false;
+    if (s0 >= hi) return hi;
+    const uint64_t q0 = g + s0, q1 = g + ((hi - s0 < W) ? hi : s0 + W);
+    const uint64_t c0 = q0 & ~3ull;
+    uint32_t lt = 0;
+    bool eq = false;
+#pragma unroll
+    for (int c = 0; c < (int)(W / 4) + 1; c++) {
+        const uint64_t e = c0 + 4ull * (uint64_t)c;
+        if (e < q1) {
+            const uint4 v = load_chunk(P, pool, e);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t q = e + (uint64_t)j;
+                const bool in = q >= q0 && q < q1;
+                lt += (in && w[j] < x) ? 1u : 0u;
+                eq |= in && w[j] == x;
+            }
+        }
+    }
+    found = eq;
+    return s0 + lt;
+}
+
 constexpr uint64_t kNoPos = ~0ull;
 
 // Pool index of x in B[g0, g1), or kNoPos, using idx[k] = B[32k] (every 32nd pool element): a
@@ -179,7 +262,7 @@ __device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__
     return hit;
 }
 
-template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, bool GALLOP>
+template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, int SMODE>
 __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
     static_assert(KS >= 1 && KS <= 31, "lane masks hold KS bits");
@@ -407,7 +490,8 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 uint32_t lstate = 0;
                 if (lb != kNotStaged) {
                     auto arr = [&](uint32_t i) { return sm.lkey[lb + (i - lo_s)]; };
-                    pos = lower_bound_from<GALLOP>(arr, cur, hi_s, x);
+                    if constexpr (SMODE == SM_BLOCK) pos = lower_bound_block<kBlockLds>(arr, cur, hi_s, x);
+                    else pos = lower_bound_from<SMODE == SM_GALLOP>(arr, cur, hi_s, x);
                     found = pos < hi_s && arr(pos) == x;
                     if constexpr (BSR)
                         if (found) lstate = sm.lst[lb + (pos - lo_s)];
@@ -423,10 +507,15 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     if constexpr (BSR)
                         if (found) lstate = in.Bs[q];
                 } else {
-                    const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
-                    auto arr = [&](uint32_t i) { return L[i]; };
-                    pos = lower_bound_from<GALLOP>(arr, cur, hi_s, x);
-                    found = pos < hi_s && L[pos] == x;
+                    if constexpr (SMODE == SM_BLOCK) {
+                        pos = lower_bound_block_pool(large_is_a ? in.A : in.B, large_is_a ? in.a_pool : in.b_pool,
+                                                     sm.seg_lbase[s], cur, hi_s, x, found);
+                    } else {
+                        const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
+                        auto arr = [&](uint32_t i) { return L[i]; };
+                        pos = lower_bound_from<SMODE == SM_GALLOP>(arr, cur, hi_s, x);
+                        found = pos < hi_s && L[pos] == x;
+                    }
                     if constexpr (BSR)
                         if (found) lstate = (large_is_a ? in.As : in.Bs)[sm.seg_lbase[s] + pos];
                 }
@@ -437,7 +526,7 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                         const uint32_t *__restrict__ X = in.xsets[j];
                         auto xarr = [&](uint32_t i) { return X[i]; };
                         const uint32_t xh = sm.xhi[j];
-                        const uint32_t xp = lower_bound_from<GALLOP>(xarr, xcur[j], xh, x);
+                        const uint32_t xp = lower_bound_from<SMODE != SM_BINARY>(xarr, xcur[j], xh, x);
                         xcur[j] = xp;
                         found = xp < xh && X[xp] == x;
                     }

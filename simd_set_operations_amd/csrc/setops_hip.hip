@@ -68,7 +68,15 @@ constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this ma
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
 constexpr int kPlaceQ = 8;  // words per lane per round of the placement kernel
 
-enum Kind { KIND_MERGE = 0, KIND_GALLOP = 1, KIND_BINARY = 2, KIND_SHUFFLE = 3, KIND_QFILTER = 4, KIND_ADAPTIVE = 5 };
+enum Kind {
+    KIND_MERGE = 0,
+    KIND_GALLOP = 1,
+    KIND_BINARY = 2,
+    KIND_SHUFFLE = 3,
+    KIND_QFILTER = 4,
+    KIND_ADAPTIVE = 5,
+    KIND_BLOCK_GALLOP = 6,
+};
 
 struct AlgoEntry {
     const char *name;
@@ -85,11 +93,20 @@ const AlgoEntry kAlgos[] = {
     {"hip_galloping", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_GALLOP, false},
     {"hip_galloping_bsr", SETOPS_ALGO_BSR, KIND_GALLOP, true},
     {"hip_binary_search", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_BINARY, false},
+    // simd_galloping (galloping_sse/avx2/avx512, simd_galloping.rs:19-193): gallop over blocks,
+    // then compare the one block that can hold the probe in full
+    {"hip_simd_galloping", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_BLOCK_GALLOP, false},
+    {"hip_simd_galloping_bsr", SETOPS_ALGO_BSR, KIND_BLOCK_GALLOP, true},
     // small_adaptive (adaptive.rs:107-147): k sets in one pass, each element of the first set
     // galloped through the others in turn; on two sets it is galloping
     {"hip_small_adaptive", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_ADAPTIVE, false},
     {"hip_shuffling", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_SHUFFLE, false},
     {"hip_shuffling_bsr", SETOPS_ALGO_BSR, KIND_SHUFFLE, true},
+    // broadcast_sse/avx2/avx512 (broadcast.rs): each B lane splatted against the A block.  In one
+    // GPU lane's registers the splats and shuffling's rotations are the same 16 compares of a 4x4
+    // block, so the entry runs the block-compare tile.
+    {"hip_broadcast", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_SHUFFLE, false},
+    {"hip_broadcast_bsr", SETOPS_ALGO_BSR, KIND_SHUFFLE, true},
     {"hip_qfilter", SETOPS_ALGO_TWOSET | SETOPS_ALGO_KSET, KIND_QFILTER, false},
     {"hip_qfilter_bsr", SETOPS_ALGO_BSR, KIND_QFILTER, true},
 };
@@ -369,11 +386,11 @@ void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
                            h->in, h->plan, out, (uint32_t)t0);
 }
 
-template <bool BSR, bool CO, bool GALLOP>
+template <bool BSR, bool CO, int SMODE>
 void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((search_tile_kernel<kNT, kKS, kMaxSeg, kCapB, BSR, CO, GALLOP>),
+        hipLaunchKernelGGL((search_tile_kernel<kNT, kKS, kMaxSeg, kCapB, BSR, CO, SMODE>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
 }
@@ -395,8 +412,9 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
         else launch_merge_t<BSR, CO, false, LP_QFILTER>(h, out, stream);
         break;
     case KIND_GALLOP:
-    case KIND_ADAPTIVE: launch_search_t<BSR, CO, true>(h, out, stream); break;
-    case KIND_BINARY: launch_search_t<BSR, CO, false>(h, out, stream); break;
+    case KIND_ADAPTIVE: launch_search_t<BSR, CO, SM_GALLOP>(h, out, stream); break;
+    case KIND_BINARY: launch_search_t<BSR, CO, SM_BINARY>(h, out, stream); break;
+    case KIND_BLOCK_GALLOP: launch_search_t<BSR, CO, SM_BLOCK>(h, out, stream); break;
     }
 }
 

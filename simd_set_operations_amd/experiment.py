@@ -19,8 +19,9 @@ Mirrors the reference's benchmark crate:
 
 Algorithm names resolve to GPU entries: a registered name is used as is, otherwise ``hip_`` + the
 name, with the reference's SIMD-width and ``_branch`` suffixes dropped (``shuffling_avx512_branch``
--> ``hip_shuffling``: vector width and branching are CPU notions).  A name with no GPU entry is reported and skipped, as the
-reference skips an unknown algorithm.  Generation uses a seeded PCG64 stream per datafile instead
+-> ``hip_shuffling``: vector width and branching are CPU notions).  ``galloping_sse/avx2/avx512``
+are the reference's SIMD galloping (simd_galloping.rs) and resolve to ``hip_simd_galloping``.  A
+name with no GPU entry is reported and skipped, as the reference skips an unknown algorithm.  Generation uses a seeded PCG64 stream per datafile instead
 of the reference's thread-local RNG, so datasets are reproducible.
 """
 import argparse
@@ -276,6 +277,8 @@ def gpu_entry_name(name, registered):
     for suffix in SIMD_SUFFIXES:
         if base.endswith(suffix):
             base = base[:-len(suffix)]
+            if base == "galloping":  # galloping_sse/avx2/avx512 name simd_galloping.rs's entries
+                base = "simd_galloping"
     cand = "hip_" + base + ("_bsr" if bsr else "")
     return cand if cand in registered else None
 

@@ -16,7 +16,8 @@ def test_cpp_program_is_built_and_lists_the_registry():
     assert r.returncode == 0, r.stdout
     names = {line.split()[0] for line in r.stdout.splitlines()}
     for n in ("hip_branchless_merge", "hip_galloping", "hip_binary_search", "hip_shuffling", "hip_qfilter",
-              "hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr"):
+              "hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr",
+              "hip_simd_galloping", "hip_simd_galloping_bsr", "hip_broadcast", "hip_broadcast_bsr"):
         assert n in names
 
 

@@ -189,4 +189,9 @@ def test_gpu_entry_names():
     assert ex.gpu_entry_name("hip_galloping", reg) == "hip_galloping"
     assert ex.gpu_entry_name("shuffling_avx2_bsr_branch", reg) == "hip_shuffling_bsr"
     assert ex.gpu_entry_name("shuffling_sse_branch", reg) == "hip_shuffling"
+    reg2 = reg | {"hip_simd_galloping", "hip_simd_galloping_bsr", "hip_broadcast_bsr"}
+    assert ex.gpu_entry_name("galloping_avx512", reg2) == "hip_simd_galloping"
+    assert ex.gpu_entry_name("galloping_sse_bsr", reg2) == "hip_simd_galloping_bsr"
+    assert ex.gpu_entry_name("galloping", reg2) == "hip_galloping"
+    assert ex.gpu_entry_name("broadcast_avx2_bsr_branch", reg2) == "hip_broadcast_bsr"
     assert ex.gpu_entry_name("croaring", reg) is None

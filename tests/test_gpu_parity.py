@@ -15,8 +15,9 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
 PLAIN = ["hip_branchless_merge", "hip_galloping", "hip_binary_search", "hip_shuffling", "hip_qfilter",
-         "hip_small_adaptive"]
-BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr"]
+         "hip_small_adaptive", "hip_simd_galloping", "hip_broadcast"]
+BSR = ["hip_branchless_merge_bsr", "hip_galloping_bsr", "hip_shuffling_bsr", "hip_qfilter_bsr", "hip_simd_galloping_bsr",
+       "hip_broadcast_bsr"]
 
 
 def announce(msg):

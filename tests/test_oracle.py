@@ -11,8 +11,10 @@ import oracle
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "pairs.npz")
 TWOSET = ["naive_merge", "branchless_merge", "galloping", "binary_search", "shuffling_sse", "shuffling_avx2",
-          "shuffling_avx512", "qfilter"]
-BSR_KINDS = ["branchless_merge_bsr", "galloping_bsr", "shuffling_bsr", "qfilter_bsr"]
+          "shuffling_avx512", "qfilter", "broadcast_sse", "galloping_sse", "galloping_avx512"]
+BSR_KINDS = ["branchless_merge_bsr", "galloping_bsr", "shuffling_bsr", "qfilter_bsr", "broadcast_bsr",
+             "simd_galloping_bsr"]
+WIDTHS = {"shuffling_bsr": (4, 8, 16), "broadcast_bsr": (4, 8, 16), "simd_galloping_bsr": (4, 8, 16)}
 
 
 def golden():
@@ -104,6 +106,29 @@ def test_invariant_trio_skewed(algo):
         assert prop_intersection_correct(r, [small, large])
 
 
+@pytest.mark.parametrize("algo", ["galloping_sse", "galloping_avx512"])
+def test_simd_galloping_block_walk(algo):
+    """simd_galloping.rs:64-107: the probe side passes every full block of the larger side while it
+    is itself at least a block long (the two sides swap), probes that land in each quarter of a
+    block, and a larger side shorter than a block (the merge tail only)."""
+    cases = [
+        (np.arange(5000, 5700), np.concatenate([np.arange(0, 1000), np.arange(5100, 5110)])),
+        (np.arange(0, 40000, 7), np.arange(0, 40000, 3)),
+        (np.array([3, 130, 260, 390, 511]), np.arange(0, 512)),
+        (np.array([5, 9, 100]), np.arange(0, 100, 2)),
+        (np.array([], np.uint32), np.arange(0, 1000)),
+    ]
+    for a, b in cases:
+        a, b = np.asarray(a, np.uint32), np.asarray(b, np.uint32)
+        exp = np.intersect1d(a, b).astype(np.uint32)
+        assert np.array_equal(oracle.intersect2(algo, a, b), exp)
+        assert np.array_equal(oracle.intersect2(algo, b, a), exp)
+    # per-quarter probes against a brute-force membership count over one block of 32 W elements
+    big = np.arange(0, 4096, 2, dtype=np.uint32)
+    probes = np.arange(0, 4096, 5, dtype=np.uint32)
+    assert oracle.count2(algo, probes, big) == sum(1 for x in probes.tolist() if x % 2 == 0)
+
+
 def test_binary_search_semantics():
     # galloping.rs:110-131: index of the target, else the insertion point lo
     s = np.array([2, 4, 6, 8], np.uint32)
@@ -169,7 +194,7 @@ def test_bsr_intersections(kind):
         a, b = sorted_set(rng, int(rng.integers(0, 3000)), u), sorted_set(rng, int(rng.integers(0, 3000)), u)
         ba, sa = oracle.bsr_from_sorted(a)
         bb, sb = oracle.bsr_from_sorted(b)
-        for w in (4, 8, 16) if kind == "shuffling_bsr" else (4,):
+        for w in WIDTHS.get(kind, (4,)):
             ob, os_ = oracle.bsr_intersect(kind, ba, sa, bb, sb, width=w)
             exp = np.intersect1d(a, b).astype(np.uint32)
             eb, es = oracle.bsr_from_sorted(exp)
