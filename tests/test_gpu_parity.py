@@ -351,7 +351,7 @@ def test_gather_batch_bsr(name):
 
 
 @pytest.mark.parametrize("indexed", [False, True])
-@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_branchless_merge"])
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_branchless_merge", "hip_simd_galloping"])
 def test_skewed_shared_large_set(name, indexed):
     """The skewed workload's shape at reduced size: one large set shared by every pair (searched
     in global memory when its narrowed window does not fit LDS) against many small sets, each with
@@ -383,7 +383,8 @@ def test_skewed_shared_large_set(name, indexed):
     gb.close()
 
 
-@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_galloping_bsr"])
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search", "hip_galloping_bsr", "hip_simd_galloping",
+                                  "hip_simd_galloping_bsr"])
 def test_indexed_b_pool_ragged_ranges(name):
     """B-pool index: B ranges that start and end anywhere inside long sets (so index lines straddle
     range ends and neighbouring sets), empty ranges, and a range that ends at the pool's last
