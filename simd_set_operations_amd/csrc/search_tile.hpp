@@ -11,8 +11,9 @@
 //   3. stage the narrowed larger windows in LDS while they fit (a prefix of the segments); the
 //      rest are searched in global memory,
 //   4. each lane looks up its KS probes, in order, from where the previous probe landed:
-//      galloping (exponential then binary, galloping.rs:7-30) or binary search over the rest of
-//      the window (binary_search_intersect, galloping.rs:32-49),
+//      galloping (exponential then binary, galloping.rs:7-30), binary search over the rest of
+//      the window (binary_search_intersect, galloping.rs:32-49) or block galloping with a full
+//      compare of the last block (simd_galloping.rs:49-107),
 //   5. block scan of hits, results (probe values; BSR: base and state AND) staged in order,
 //      tile and per-pair counts -- the same outputs as the merge tile kernel.
 #pragma once
