@@ -4,7 +4,8 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker (or the timed CPU baseline) -- never as the thing measured or shipped.
 
 * ``liboracle.so``            -- setops_oracle.c, one C function per reference function (each
-  cites the reference file:line it restates).
+  cites the reference file:line it restates), and roaring.c, a restatement of the croaring
+  comparator the reference's benchmark times beside its own algorithms.
 * ``_ref/libqfilter_ref.so``  -- the reference's own setops/ffi/qfilter/qfilter.cpp, compiled
   from its source by oracle/Makefile (absent when the reference checkout was unavailable at
   build time).
@@ -21,6 +22,8 @@ Pinning (what checks each oracle function, see tests/test_oracle.py):
       decode(result) == plain intersection, the compiled reference intersect_qfilter_bsr_b4_v2.
   bsr_from_sorted / bsr_to_sorted: round trip, popcount == length, bases strictly increasing.
   svs: left fold of numpy.intersect1d == k-way intersection (reduce over all sets).
+  roaring_intersect / roaring_svs (roaring.c, the croaring comparator): numpy.intersect1d and
+      brute force, on sets that exercise array and bitmap containers across several high keys.
 """
 import ctypes
 import os
@@ -108,6 +111,17 @@ def lib():
         L.oracle_small_adaptive.argtypes = [_vp, _vp, _sz, _vp, _vp]
         L.oracle_batch_intersect.restype = _c.c_int
         L.oracle_batch_intersect.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]
+        # roaring.c: the croaring comparator
+        L.roaring_intersect.restype = _sz
+        L.roaring_intersect.argtypes = [_vp, _sz, _vp, _sz, _vp]
+        L.roaring_svs.restype = _sz
+        L.roaring_svs.argtypes = [_vp, _vp, _sz, _vp]
+        L.roaring_batch_create.restype = _vp
+        L.roaring_batch_create.argtypes = [_vp, _vp, _vp, _vp, _c.c_uint64]
+        L.roaring_batch_free.restype = None
+        L.roaring_batch_free.argtypes = [_vp]
+        L.roaring_batch_run.restype = _c.c_double
+        L.roaring_batch_run.argtypes = [_vp, _c.c_int, _c.c_int, _vp]
         L.oracle_batch_intersect_gather.restype = _c.c_int
         L.oracle_batch_intersect_gather.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
                                                     _vp]
@@ -295,6 +309,57 @@ def ref_qfilter_bsr(ba, sa, bb, sb, v2=True):
     f = R.intersect_qfilter_bsr_b4_v2 if v2 else R.intersect_qfilter_bsr_b4
     n = f(_p(A), _p(As), len(ba), _p(B), _p(Bs), len(bb), _p(ob), _p(os_))
     return ob[:n].copy(), os_[:n].copy()
+
+
+def roaring_intersect(a, b):
+    """The croaring comparator (roaring.c): a & b through roaring bitmaps, decoded ascending."""
+    a, b = _u32(a), _u32(b)
+    out = np.zeros(max(min(a.size, b.size), 1), np.uint32)
+    n = lib().roaring_intersect(_p(a), a.size, _p(b), b.size, _p(out))
+    return out[:n].copy()
+
+
+def roaring_svs(sets):
+    """time_croaring_svs's fold (harness.rs:226-253): the first set's bitmap ANDed in place with
+    each further set; decoded ascending."""
+    arrs = [_u32(s) for s in sets]
+    k = len(arrs)
+    out = np.zeros(max(arrs[0].size, 1), np.uint32)
+    ptrs = (_vp * k)(*[a.ctypes.data if a.size else None for a in arrs])
+    lens = (_sz * k)(*[a.size for a in arrs])
+    n = lib().roaring_svs(_c.cast(ptrs, _vp), _c.cast(lens, _vp), k, _p(out))
+    return out[:n].copy()
+
+
+class RoaringBatch:
+    """The croaring comparator over a CSR batch: every set's roaring bitmap built once (the
+    harness's prepare step, outside any timed region), then each pair's and_cardinality (or the
+    materialised and) on host threads."""
+
+    def __init__(self, A, a_off, B, b_off):
+        self._keep = [np.ascontiguousarray(A, np.uint32), np.ascontiguousarray(a_off, np.uint64),
+                      np.ascontiguousarray(B, np.uint32), np.ascontiguousarray(b_off, np.uint64)]
+        A_, ao, B_, bo = self._keep
+        self.P = ao.size - 1
+        self.h = lib().roaring_batch_create(_p(A_), _p(ao), _p(B_), _p(bo), self.P)
+        if not self.h:
+            raise MemoryError("roaring batch build failed")
+        self.counts = np.zeros(max(self.P, 1), np.uint32)
+
+    def run(self, nthreads, materialize=False):
+        """Wall seconds of one pass; per-pair counts in self.counts."""
+        t = lib().roaring_batch_run(self.h, int(nthreads), 1 if materialize else 0, _p(self.counts))
+        if t < 0:
+            raise ValueError("bad roaring batch arguments")
+        return t
+
+    def close(self):
+        if self.h:
+            lib().roaring_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def _aligned(src, n):

@@ -279,3 +279,67 @@ def test_batch_runner_matches_pairwise():
         exp = np.intersect1d(As[p], Bs[p])
         assert counts[p] == exp.size
         assert np.array_equal(data[int(off[p]):int(off[p + 1])], exp)
+
+
+# -- the croaring comparator (oracle/roaring.c) ----------------------------------------------------
+def _roaring_cases(rng):
+    """Sets whose roaring forms use array containers (<= 4096 values under a high key), bitmap
+    containers (more), several high keys, keys present in only one set, and empty sets."""
+    cases = []
+    for _ in range(30):
+        u = int(rng.choice([1 << 12, 1 << 16, 1 << 18, 1 << 22]))
+        na, nb = int(rng.integers(0, 30000)), int(rng.integers(0, 30000))
+        cases.append((sorted_set(rng, na, u), sorted_set(rng, nb, u)))
+    dense = np.arange(0, 1 << 17, 3, dtype=np.uint32)  # two bitmap containers
+    cases.append((dense, np.arange(0, 1 << 17, 2, dtype=np.uint32)))
+    cases.append((dense, np.array([3, 4, 65538, 65539, 131070], np.uint32)))  # array & bitmap
+    cases.append((np.arange(0, 4096, dtype=np.uint32) * 16, dense))  # 4096-value array container
+    cases.append((np.zeros(0, np.uint32), dense))
+    cases.append((np.array([0xFFFFFFFF, 5], np.uint32)[::-1].copy(), np.array([5, 0xFFFFFFFF], np.uint32)))
+    return cases
+
+
+def test_roaring_against_numpy():
+    rng = np.random.Generator(np.random.PCG64(23))
+    for a, b in _roaring_cases(rng):
+        exp = np.intersect1d(a, b).astype(np.uint32)
+        assert np.array_equal(oracle.roaring_intersect(a, b), exp)
+        assert np.array_equal(oracle.roaring_intersect(b, a), exp)
+
+
+def test_roaring_skewed_arrays_gallop():
+    # array & array with one side at least 64x smaller takes the galloping path
+    big = np.arange(0, 4000, dtype=np.uint32)
+    small = np.array([0, 17, 63, 1999, 3999, 4000], np.uint32)
+    assert oracle.roaring_intersect(small, big).tolist() == [0, 17, 63, 1999, 3999]
+
+
+def test_roaring_brute_force_tiny():
+    u = 6
+    for ma in range(1 << u):
+        a = np.array([x for x in range(u) if ma >> x & 1], np.uint32)
+        for mb in range(0, 1 << u, 5):
+            b = np.array([x for x in range(u) if mb >> x & 1], np.uint32)
+            assert oracle.roaring_intersect(a, b).tolist() == [x for x in range(u) if (ma & mb) >> x & 1]
+
+
+def test_roaring_svs_and_batch():
+    rng = np.random.Generator(np.random.PCG64(29))
+    sets = [sorted_set(rng, int(rng.integers(5000, 40000)), 1 << 17) for _ in range(4)]
+    exp = sets[0]
+    for s in sets[1:]:
+        exp = np.intersect1d(exp, s)
+    assert np.array_equal(oracle.roaring_svs(sets), exp.astype(np.uint32))
+    # batch: counts equal the pairwise oracle's, count-only and materialised, 1 and 3 threads
+    a_sets = [sorted_set(rng, int(rng.integers(0, 9000)), 1 << 16) for _ in range(150)]
+    b_sets = [sorted_set(rng, int(rng.integers(0, 9000)), 1 << 16) for _ in range(150)]
+    A = np.concatenate(a_sets)
+    B = np.concatenate(b_sets)
+    a_off = np.concatenate([[0], np.cumsum([x.size for x in a_sets])]).astype(np.uint64)
+    b_off = np.concatenate([[0], np.cumsum([x.size for x in b_sets])]).astype(np.uint64)
+    want = np.array([np.intersect1d(x, y).size for x, y in zip(a_sets, b_sets)], np.uint32)
+    rb = oracle.RoaringBatch(A, a_off, B, b_off)
+    for nthreads, mat in ((1, False), (3, True)):
+        assert rb.run(nthreads, materialize=mat) >= 0
+        assert np.array_equal(rb.counts[:150], want)
+    rb.close()
