@@ -316,6 +316,81 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     return res
 
 
+def _median_us(fn, warmup_s=1.0, min_runs=5, run_s=1.0):
+    """Harness::time's shape (benchmark/src/timer/harness.rs:35-62): calls for ~warmup_s untimed,
+    then at least min_runs timed calls (more while run_s lasts); the median in microseconds."""
+    t_end = time.perf_counter() + warmup_s
+    while time.perf_counter() < t_end:
+        fn()
+    times = []
+    t_end = time.perf_counter() + run_s
+    while len(times) < min_runs or (time.perf_counter() < t_end and len(times) < 10000):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)) * 1e6, len(times)
+
+
+def pair_leg(torch, with_cpu):
+    """The single plain pair: 2^16 + 2^16 sorted u32 over 2^20 with 19,661 shared.  The CPU
+    branchless_merge restatement and the reference's own QFilter (one thread), and the GPU plain
+    drop-in through the C ABI -- host pointers (copies included) and device-resident sets."""
+    import ctypes
+
+    from simd_set_operations_amd import _native, datagen
+    from simd_set_operations_amd.intersect import algo_id
+
+    a, b = datagen.plain_pair()
+    E = int(a.size + b.size)
+    res = {"elements": E, "results": int(datagen.PLAIN_PAIR["shared"])}
+    lib = _native.hip()
+    algo = algo_id("hip_branchless_merge")
+    out = np.zeros(min(a.size, b.size) + 4, np.uint32)
+    n = ctypes.c_size_t()
+    vp = ctypes.c_void_p
+
+    def gpu_host():
+        _native.check(lib.setops_hip_intersect2(algo, vp(a.ctypes.data), a.size, vp(b.ctypes.data), b.size,
+                                                vp(out.ctypes.data), out.size, ctypes.byref(n)))
+
+    gpu_host()
+    if n.value != res["results"]:
+        log(f"plain pair: {n.value} results, expected {res['results']}")
+        sys.exit(3)
+    us, runs = _median_us(gpu_host)
+    res["gpu_host_pointers"] = {"us": us, "runs": runs, "gelems": E / us / 1e3}
+    dA = torch.from_numpy(a.view(np.int32)).cuda()
+    dB = torch.from_numpy(b.view(np.int32)).cuda()
+    dO = torch.empty(out.size, dtype=torch.int32, device="cuda")
+    dT = torch.zeros(2, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def gpu_dev():
+        _native.check(lib.setops_hip_intersect2_dev(algo, dA.data_ptr(), None, a.size, dB.data_ptr(), None, b.size,
+                                                    dO.data_ptr(), None, out.size, dT.data_ptr(), stream.cuda_stream))
+        stream.synchronize()
+
+    us, runs = _median_us(gpu_dev)
+    res["gpu_device_resident"] = {"us": us, "runs": runs, "gelems": E / us / 1e3}
+    if with_cpu:
+        import oracle
+
+        us, runs = _median_us(lambda: oracle.branchless_merge(a, b))
+        res["cpu_port_branchless_merge"] = {"us": us, "runs": runs, "gelems": E / us / 1e3, "threads": 1}
+        if oracle.ref_available():
+            R = oracle.ref()
+            Aa = oracle._aligned(a, a.size + 8)
+            Ba = oracle._aligned(b, b.size + 8)
+            Oa = oracle._aligned(np.zeros(0, np.uint32), min(a.size, b.size) + 8)
+            args = (vp(Aa.ctypes.data), a.size, vp(Ba.ctypes.data), b.size, vp(Oa.ctypes.data))
+            if R.intersect_qfilter_uint_b4_v2(*args) != res["results"]:
+                log("plain pair: reference qfilter count differs")
+                sys.exit(3)
+            us, runs = _median_us(lambda: R.intersect_qfilter_uint_b4_v2(*args))
+            res["cpu_reference_qfilter_c"] = {"us": us, "runs": runs, "gelems": E / us / 1e3, "threads": 1}
+    return res
+
+
 def bsr_leg(torch, A, a_off, B, b_off, out_total, steps, warmup):
     """The uneven batch in the BSR form (base = v >> 5, 32-bit state; setops/src/bsr.rs:140-164),
     encoded outside the timed region as the reference's harness does (harness.rs:121-122), through
@@ -460,6 +535,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the served k-set leg")
     ap.add_argument("--no-kset", action="store_true", help="skip the host-pointer k-set leg")
     ap.add_argument("--no-bsr", action="store_true", help="skip the BSR form of the uneven batch")
+    ap.add_argument("--no-pair", action="store_true", help="skip the single plain pair (2^16 + 2^16)")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
     ap.add_argument("--corpus-mode", choices=("iid", "rank"), default="iid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -587,6 +663,9 @@ def main():
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
         log(f"cpu: {res}")
+    if not args.no_pair:
+        extra["plain_pair"] = pair_leg(torch, not args.no_baselines)
+        log(f"plain pair: {extra['plain_pair']}")
     if not args.no_bsr:
         extra["bsr"] = bsr_leg(torch, A, a_off, B, b_off, out_total, args.steps, args.warmup)
     if not args.no_kset:
