@@ -347,12 +347,17 @@ uint32_t grid_for(uint64_t n, uint64_t per_block)
 int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
-    hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
-                       h->plan.gsum2, (uint64_t)ng, 0, h->d_tot);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((units_scan_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan, out);
+    // up to kScanGroup pairs (serving steps, small gather batches): the scan is one workgroup
+    if (ng > 1) {
+        hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in,
+                           h->plan);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                           h->plan.gsum2, (uint64_t)ng, 0, h->d_tot);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL((units_scan_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan, out,
+                       ng == 1 ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return SETOPS_OK;
 }

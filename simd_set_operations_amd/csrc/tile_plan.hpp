@@ -257,9 +257,10 @@ __global__ __launch_bounds__(NT) void groups_scan_kernel(unsigned long long *gsu
     }
 }
 
-// Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.
+// Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.  With
+// `one_group` (a batch of at most NT pairs) the base is 0 and the group-sum kernels do not run.
 template <int NT>
-__global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out)
+__global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out, int one_group)
 {
     __shared__ unsigned long long wsum[NT / kWave];
     const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan p
     }
     if (l == kWave - 1) wsum[w] = x;
     __syncthreads();
-    unsigned long long pre = plan.gsum[blockIdx.x];
+    unsigned long long pre = one_group ? 0ull : plan.gsum[blockIdx.x];
     for (int k = 0; k < (int)w; k++) pre += wsum[k];
     if (p < in.P) {
         const uint64_t u = pre + x - v, un = pre + x;
