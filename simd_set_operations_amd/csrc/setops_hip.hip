@@ -453,31 +453,37 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev->second, stream));
     if (h->plan.direct) return SETOPS_OK;  // results and totals are final
-    // scan: with few groups one kernel (a workgroup adds the earlier groups' tile counts itself);
-    // otherwise group sums, one-block scan of the group sums (writes the totals), per-group scan
-    const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
-    const int direct = ngroups <= kScanDirectGroups ? 1 : 0;
-    TileOut so = out;
-    if (!so.totals) so.totals = h->d_tot;
-    if (!direct) {
-        hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
-                           h->plan.gsum2, (uint64_t)ngroups, 1, so.totals);
+    // scan: with few tiles the placement kernel scans the tile counts itself; with few groups one
+    // kernel (a workgroup adds the earlier groups' tile counts itself); otherwise group sums,
+    // one-block scan of the group sums (writes the totals), per-group scan
+    const bool place = !count_only || out.out_off;
+    const int fused = (place && h->plan.ntiles <= kFuseScanTiles) ? 1 : 0;
+    if (!fused) {
+        const uint32_t ngroups = (h->plan.ntiles + kScanGroup - 1) / kScanGroup;
+        const int direct = ngroups <= kScanDirectGroups ? 1 : 0;
+        TileOut so = out;
+        if (!so.totals) so.totals = h->d_tot;
+        if (!direct) {
+            hipLaunchKernelGGL((reduce_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream,
+                               h->plan);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
+                               h->plan.gsum2, (uint64_t)ngroups, 1, so.totals);
+            HIP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, so,
+                           direct);
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL((scan_tiles_kernel<kScanGroup>), dim3(ngroups), dim3(kScanGroup), 0, stream, h->plan, so,
-                       direct);
-    HIP_TRY(hipGetLastError());
     // write: one workgroup per group of kPlaceTPW tiles
-    if (!count_only || out.out_off) {
+    if (place) {
         const uint32_t gp = grid_for((h->plan.ntiles + kPlaceTPW - 1) / kPlaceTPW, 1);
         if (h->bsr)
             hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan,
-                               out);
+                               out, fused);
         else
             hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, false>), dim3(gp), dim3(256), 0, stream, h->in,
-                               h->plan, out);
+                               h->plan, out, fused);
         HIP_TRY(hipGetLastError());
     }
     return SETOPS_OK;

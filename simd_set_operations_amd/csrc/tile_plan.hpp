@@ -447,23 +447,77 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
 // reads the word from that tile's staging slot -- with all of its loads in flight before its
 // stores, and writes the offsets of the pairs that start in the group (the tile holding a pair's
 // first unit recorded the pair's in-tile result offset).
+// With `fused_scan` (at most kFuseScanTiles tiles) no tile scan ran before: each workgroup sums
+// the tile counts before its group itself, writes its tiles' offsets, and the workgroup of the
+// last group writes the totals, the capacity flag and the end offset.
+constexpr uint32_t kFuseScanTiles = 1024;
+
 template <int NT, int Q, bool BSR>
-__global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out)
+__global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out, int fused_scan)
 {
     constexpr uint32_t TPW = kPlaceTPW;
     static_assert((TPW & (TPW - 1)) == 0 && TPW < (uint32_t)NT, "a log2(TPW)-step search, one thread per tile offset");
+    static_assert(kFuseScanTiles <= 4 * (uint32_t)NT, "four tile counts per thread");
     __shared__ uint32_t toff[TPW + 1];
+    __shared__ uint32_t gcnt[TPW];
+    __shared__ unsigned long long red[NT / kWave][2];
     const uint32_t tid = threadIdx.x;
     const uint64_t ngrp = (plan.ntiles + TPW - 1) / TPW;
     const uint32_t sw = plan.stage_words;
     for (uint64_t w = blockIdx.x; w < ngrp; w += gridDim.x) {
         const uint64_t tw0 = w * TPW;
         const uint32_t ntw = (plan.ntiles - tw0 < TPW) ? (uint32_t)(plan.ntiles - tw0) : TPW;
-        const uint64_t base = plan.tile_off[tw0];
+        const bool last_group = w + 1 == ngrp;
+        uint64_t base;
+        if (fused_scan) {
+            // results before the group (and, for the last group, the elements of every tile)
+            unsigned long long b = 0, el = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = tid + (uint32_t)k * NT;
+                if (i < tw0) b += plan.tile_count[i];
+                if (last_group && i < plan.ntiles) el += plan.tile_elems[i];
+            }
+            if (tid < ntw) gcnt[tid] = plan.tile_count[tw0 + tid];
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                b += __shfl_xor(b, d, kWave);
+                el += __shfl_xor(el, d, kWave);
+            }
+            if (lane_id() == 0) {
+                red[tid / kWave][0] = b;
+                red[tid / kWave][1] = el;
+            }
+            __syncthreads();
+            base = 0;
+            el = 0;
+            for (int k = 0; k < NT / kWave; k++) {
+                base += red[k][0];
+                el += red[k][1];
+            }
+            if (tid <= TPW) {
+                uint32_t o = 0;
+                for (uint32_t j = 0; j < tid && j < ntw; j++) o += gcnt[j];
+                toff[tid] = (tid <= ntw) ? o : ~0u;
+                if (tid < ntw) plan.tile_off[tw0 + tid] = base + o;
+                if (last_group && tid == ntw) {
+                    const unsigned long long total = base + o;
+                    plan.tile_off[plan.ntiles] = total;
+                    if (out.totals) {
+                        out.totals[0] = total;
+                        out.totals[1] = el;
+                    }
+                    if (out.out && total > out.cap && out.error) atomicOr(out.error, 1u);
+                    if (out.out_off) out.out_off[in.P] = total;
+                }
+            }
+        } else {
+            base = plan.tile_off[tw0];
+            // toff[k]: tile k's first word, relative to the group's (< TPW * stage slot); past the
+            // group's end: all ones, so the search never stops there
+            if (tid <= TPW) toff[tid] = (tid <= ntw) ? (uint32_t)(plan.tile_off[tw0 + tid] - base) : ~0u;
+        }
         const uint64_t pf = plan.tile_first[tw0], pl = plan.tile_last[tw0 + ntw - 1];
-        // toff[k]: tile k's first word, relative to the group's (< TPW * stage slot); past the
-        // group's end: all ones, so the search never stops there
-        if (tid <= TPW) toff[tid] = (tid <= ntw) ? (uint32_t)(plan.tile_off[tw0 + tid] - base) : ~0u;
         // the first pairs' prefix entries and in-tile offsets load with the tile offsets
         const uint64_t p = pf + tid;
         const bool pv = out.out_off && p <= pl;
@@ -522,9 +576,9 @@ __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan 
                 if (uq >= u0) out.out_off[q] = base + toff[(uint32_t)(uq - u0) / (uint32_t)U] + plan.pair_local[q];
             }
         }
-        __syncthreads();  // toff is rewritten by the next group
+        __syncthreads();  // toff (and red, gcnt) are rewritten by the next group
     }
-    if (out.out_off && blockIdx.x == 0 && tid == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
+    if (!fused_scan && out.out_off && blockIdx.x == 0 && tid == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
 }
 
 // ---- per-tile segment geometry -------------------------------------------------------------------
