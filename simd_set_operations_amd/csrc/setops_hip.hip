@@ -344,14 +344,10 @@ uint32_t grid_for(uint64_t n, uint64_t per_block)
 }
 
 // Unit prefix by the three-kernel scan (gather batches, search policy).
-// *split_done: the same kernel also found the tiles' starts (no plan_split_kernel needed).
-int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t stream, bool *split_done)
+int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
-    // up to kScanGroup pairs (serving steps, small gather batches): the scan is one workgroup,
-    // which with up to kFuseSplitTiles tiles also splits them
-    const int mode = ng > 1 ? 0 : (h->plan.ntiles <= kFuseSplitTiles && !h->plan.direct) ? 2 : 1;
-    *split_done = mode == 2;
+    // up to kScanGroup pairs (serving steps, small gather batches): the scan is one workgroup
     if (ng > 1) {
         hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in,
                            h->plan);
@@ -361,7 +357,7 @@ int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t strea
         HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL((units_scan_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan, out,
-                       mode);
+                       ng == 1 ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return SETOPS_OK;
 }
@@ -433,15 +429,14 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
                   std::pair<hipEvent_t, hipEvent_t> *ev)
 {
     // plan: unit prefix and tile maps, then tile starts
-    bool split_done = false;
     if (closed_form(h)) {
         hipLaunchKernelGGL(plan_csr_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
         HIP_TRY(hipGetLastError());
     } else {
-        int rc = launch_units_scan(h, out, stream, &split_done);
+        int rc = launch_units_scan(h, out, stream);
         if (rc != SETOPS_OK) return rc;
     }
-    if (!h->plan.direct && !split_done) {
+    if (!h->plan.direct) {
         hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in,
                            h->plan);
         HIP_TRY(hipGetLastError());

@@ -257,55 +257,12 @@ __global__ __launch_bounds__(NT) void groups_scan_kernel(unsigned long long *gsu
     }
 }
 
-// Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
-// A index i and B index d - i with every A[x < i] <= B[d-1-...]), or the probe index (search).
-// One thread per tile (a cooperative search costs more loads in total than it saves latency).
-__device__ __forceinline__ void split_tile(const PairBatch &in, const TilePlan &plan, uint64_t t, uint64_t p,
-                                           uint64_t up)
-{
-    const uint64_t a = in.a_beg[p], b = in.b_beg[p];
-    const uint64_t na = in.a_end[p] - a, nb = in.b_end[p] - b;
-    const uint64_t rel = t * plan.U - up;
-    if (plan.policy == POLICY_SEARCH) {
-        const uint64_t np = na < nb ? na : nb;
-        plan.tile_sa[t] = (uint32_t)(rel < np ? rel : np);
-        plan.tile_sb[t] = 0;
-        return;
-    }
-    const uint64_t d = rel < na + nb ? rel : na + nb;
-    uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
-    const uint32_t *__restrict__ A = in.A + a;
-    const uint32_t *__restrict__ B = in.B + b;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
-    }
-    plan.tile_sa[t] = (uint32_t)lo;
-    plan.tile_sb[t] = (uint32_t)(d - lo);
-    if (in.contig) {
-        // CSR: a tile's A run is [tile_ga[t], tile_ga[t+1]) -- one load for the staging loop
-        plan.tile_ga[t] = a + lo;
-        plan.tile_gb[t] = b + (d - lo);
-        if (t + 1 == plan.ntiles) {
-            plan.tile_ga[t + 1] = in.a_end[in.P - 1];
-            plan.tile_gb[t + 1] = in.b_end[in.P - 1];
-        }
-    }
-}
-
 // Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.  With
-// `one_group` (a batch of at most NT pairs) the base is 0 and the group-sum kernels do not run;
-// with one_group == 2 (also at most kFuseSplitTiles tiles) the workgroup then finds every tile's
-// start as well (split_tile), from the prefix and tile map it keeps in LDS, in place of
-// plan_split_kernel.
-constexpr uint32_t kFuseSplitTiles = 1024;
-
+// `one_group` (a batch of at most NT pairs) the base is 0 and the group-sum kernels do not run.
 template <int NT>
 __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out, int one_group)
 {
     __shared__ unsigned long long wsum[NT / kWave];
-    __shared__ unsigned long long s_uoff[NT];
-    __shared__ uint32_t s_first[kFuseSplitTiles];
     const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
     const unsigned long long v = (p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
     // 64-bit block scan: wave scan, then wave totals
@@ -326,29 +283,46 @@ __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan p
         if (p + 1 == in.P) plan.uoff[in.P] = un;
         map_tiles(plan, p, u, un);
         if (out.counts) out.counts[p] = 0;
-        if (one_group == 2) {
-            s_uoff[p] = u;
-            const uint64_t U = plan.U;
-            for (uint64_t t = (u + U - 1) / U; t * U < un && t < plan.ntiles; t++) s_first[t] = (uint32_t)p;
-        }
     }
     if (p == 0) plan_reset(out);
-    if (one_group == 2) {
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < plan.ntiles; t += NT) {
-            const uint32_t q = s_first[t];
-            split_tile(in, plan, t, q, s_uoff[q]);
-        }
-    }
 }
 
+// Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
+// A index i and B index d - i with every A[x < i] <= B[d-1-...]), or the probe index (search).
 // One thread per tile (a cooperative search costs more loads in total than it saves latency).
 __global__ void plan_split_kernel(PairBatch in, TilePlan plan)
 {
     const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < plan.ntiles; t += gsz) {
         const uint64_t p = plan.tile_first[t];
-        split_tile(in, plan, t, p, plan.uoff[p]);
+        const uint64_t a = in.a_beg[p], b = in.b_beg[p];
+        const uint64_t na = in.a_end[p] - a, nb = in.b_end[p] - b;
+        const uint64_t rel = (uint64_t)t * plan.U - plan.uoff[p];
+        if (plan.policy == POLICY_SEARCH) {
+            const uint64_t np = na < nb ? na : nb;
+            plan.tile_sa[t] = (uint32_t)(rel < np ? rel : np);
+            plan.tile_sb[t] = 0;
+            continue;
+        }
+        const uint64_t d = rel < na + nb ? rel : na + nb;
+        uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+        const uint32_t *__restrict__ A = in.A + a;
+        const uint32_t *__restrict__ B = in.B + b;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
+        }
+        plan.tile_sa[t] = (uint32_t)lo;
+        plan.tile_sb[t] = (uint32_t)(d - lo);
+        if (in.contig) {
+            // CSR: a tile's A run is [tile_ga[t], tile_ga[t+1]) -- one load for the staging loop
+            plan.tile_ga[t] = a + lo;
+            plan.tile_gb[t] = b + (d - lo);
+            if (t + 1 == plan.ntiles) {
+                plan.tile_ga[t + 1] = in.a_end[in.P - 1];
+                plan.tile_gb[t + 1] = in.b_end[in.P - 1];
+            }
+        }
     }
 }
 
