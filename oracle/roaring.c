@@ -293,20 +293,28 @@ typedef struct {
     rb_bitmap *a, *b;
 } roaring_batch;
 
+void roaring_batch_free(void *h);
+
 void *roaring_batch_create(const uint32_t *A, const uint64_t *a_off, const uint32_t *B, const uint64_t *b_off,
                            uint64_t P)
 {
     roaring_batch *rb = (roaring_batch *)calloc(1, sizeof(roaring_batch));
     if (!rb) return NULL;
-    rb->P = P;
     rb->a = (rb_bitmap *)calloc(P ? P : 1, sizeof(rb_bitmap));
     rb->b = (rb_bitmap *)calloc(P ? P : 1, sizeof(rb_bitmap));
-    if (!rb->a || !rb->b) return NULL;
-    for (uint64_t p = 0; p < P; p++) {
-        if (rb_from_sorted(A + a_off[p], a_off[p + 1] - a_off[p], &rb->a[p]) ||
-            rb_from_sorted(B + b_off[p], b_off[p + 1] - b_off[p], &rb->b[p]))
-            return NULL;
+    if (!rb->a || !rb->b) {
+        roaring_batch_free(rb);
+        return NULL;
     }
+    for (uint64_t p = 0; p < P; p++) {
+        rb->P = p + 1;  /* sets built so far (freed on failure) */
+        if (rb_from_sorted(A + a_off[p], a_off[p + 1] - a_off[p], &rb->a[p]) ||
+            rb_from_sorted(B + b_off[p], b_off[p + 1] - b_off[p], &rb->b[p])) {
+            roaring_batch_free(rb);
+            return NULL;
+        }
+    }
+    rb->P = P;
     return rb;
 }
 
