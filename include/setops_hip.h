@@ -173,7 +173,8 @@ int setops_hip_batch_set_timing(setops_hip_batch *handle, int enabled);
  * last call; resets the accumulator. */
 int setops_hip_batch_kernel_time(setops_hip_batch *handle, double *total_ms, uint64_t *runs);
 
-/* Number of tiles and the persistent grid size of the handle (for reporting). */
+/* Number of tiles, the tile kernel's grid size (one workgroup per tile) and the unit total of the
+ * handle's plan (for reporting). */
 int setops_hip_batch_info(setops_hip_batch *handle, uint64_t *ntiles, uint64_t *grid, uint64_t *units);
 
 int setops_hip_batch_destroy(setops_hip_batch *handle);
