@@ -17,7 +17,8 @@ import time
 
 import numpy as np
 
-ALGOS = ("hip_branchless_merge", "hip_shuffling", "hip_qfilter", "hip_galloping", "hip_binary_search")
+ALGOS = ("hip_branchless_merge", "hip_shuffling", "hip_qfilter", "hip_galloping", "hip_binary_search",
+         "hip_simd_galloping")
 GRID = {
     "log2_larger": [6, 8, 10, 12, 14],
     "log2_skew": [0, 3, 6, 10],
