@@ -5,8 +5,12 @@
  * UNSW-database/simd_set_operations.  Every entry point names the reference interface it
  * replaces (paths relative to that repository).  Conventions shared by all entry points:
  *
- *   - Sets are sorted, strictly increasing uint32_t arrays.  (The reference harness passes i32,
- *     benchmark/src/datafile.rs:25; for values below 2^31 the orders agree.)
+ *   - Sets are sorted, strictly increasing uint32_t arrays, or -- with SETOPS_SIGNED_I32 OR'ed into
+ *     the algorithm id (plain entries) -- sorted, strictly increasing int32_t arrays passed through
+ *     the same pointers, compared as signed.  The reference harness passes i32
+ *     (benchmark/src/timer.rs:19, benchmark/src/datafile.rs:25) and its property tests draw
+ *     negative values (setops/tests/testlib/mod.rs:11-30); its BSR form casts them to u32 first
+ *     (benchmark/src/timer/harness.rs:121-122), so BSR entries have no int32 mode.
  *   - Results are the intersection in ascending order, identical element for element to the
  *     reference's branchless_merge (setops/src/intersect/merge.rs:39-60) and svs_generic
  *     (setops/src/intersect/svs.rs:50-73).
@@ -56,6 +60,10 @@ const char *setops_hip_version(void);
 #define SETOPS_ALGO_TWOSET 1u  /* plain 2-set entry (timer.rs:75-169 try_parse_twoset) */
 #define SETOPS_ALGO_BSR 2u     /* BSR 2-set entry (timer.rs:211-259 try_parse_bsr) */
 #define SETOPS_ALGO_KSET 4u    /* usable for k sets through svs_generic (timer.rs:190-198) */
+
+/* OR'ed into the id of a plain (non-BSR) entry: the sets are int32_t, ordered as signed integers.
+ * Every entry point that takes an algorithm id accepts it; results are the same int32 values. */
+#define SETOPS_SIGNED_I32 0x10000
 
 /* Number of registered entries; ids are 0 .. count-1. */
 int setops_hip_algo_count(void);
@@ -107,6 +115,16 @@ int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t 
  * with no intermediate result between the sets of a launch. */
 int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, size_t k, uint32_t *out,
                    size_t out_cap, size_t *out_len);
+
+/* ---- int32 forms of the host-pointer calls -------------------------------------------------
+ * setops_hip_intersect2 and setops_hip_svs with SETOPS_SIGNED_I32, with int32_t pointers: the
+ * shapes the reference's harness times, Intersect2C<[i32]> (setops/src/intersect.rs:41,
+ * time_twoset_c in benchmark/src/timer/harness.rs:98-112) and the i32 k-set fold of time_svs
+ * (harness.rs:152-172, svs_generic of setops/src/intersect/svs.rs:50-73). */
+int setops_hip_intersect2_i32(int algo, const int32_t *a, size_t na, const int32_t *b, size_t nb, int32_t *out,
+                              size_t out_cap, size_t *out_len);
+int setops_hip_svs_i32(int algo, const int32_t *const *sets, const size_t *lens, size_t k, int32_t *out, size_t out_cap,
+                       size_t *out_len);
 
 /* ---- device-resident batches ----------------------------------------------------------------
  * A batch is P pairs in CSR form: pair p is (A[a_off[p] .. a_off[p+1]), B[b_off[p] .. b_off[p+1])),

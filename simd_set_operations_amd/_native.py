@@ -24,6 +24,7 @@ SETOPS_ERR_NO_DEVICE = -7
 ALGO_TWOSET = 1
 ALGO_BSR = 2
 ALGO_KSET = 4
+SIGNED_I32 = 0x10000  # OR'ed into an algorithm id: int32 sets, signed order
 
 _ERR_NAMES = {
     SETOPS_ERR_INVALID: "SETOPS_ERR_INVALID",
@@ -68,6 +69,8 @@ HIP_SIGNATURES = {
         [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _szp, _u64p],
     ),
     "setops_hip_svs": (_c.c_int, [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _szp]),
+    "setops_hip_intersect2_i32": (_c.c_int, [_c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t, _szp]),
+    "setops_hip_svs_i32": (_c.c_int, [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _szp]),
     "setops_hip_batch_create": (
         _c.c_int,
         [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _c.c_uint64, _vp, _c.c_uint64],

@@ -32,7 +32,8 @@ def to_device_u32(arr, torch):
     aligned (torch allocations are 256-byte aligned).  A device tensor is used as it is."""
     if _is_tensor(arr):
         return arr
-    a = np.ascontiguousarray(arr, dtype=np.uint32)
+    a = np.ascontiguousarray(arr)
+    a = a.view(np.uint32) if a.dtype == np.int32 else np.ascontiguousarray(a, dtype=np.uint32)
     t = torch.from_numpy(a.view(np.int32)) if a.size else torch.zeros(4, dtype=torch.int32)
     return t.to("cuda")
 
@@ -73,10 +74,19 @@ class _Batch:
         self.d_totals = torch.zeros(2, dtype=torch.int64, device="cuda")
 
 
-class DeviceBatch(_Batch):
-    """CSR batch on the GPU: pair p = (A[a_off[p]:a_off[p+1]], B[b_off[p]:b_off[p+1]])."""
+def _signed(signed, A, B):
+    """int32 mode (SETOPS_SIGNED_I32): as asked, else when a numpy pool is int32."""
+    if signed is not None:
+        return bool(signed)
+    return any(not _is_tensor(x) and np.asarray(x).dtype == np.int32 for x in (A, B))
 
-    def __init__(self, algo, A, a_off, B, b_off, A_states=None, B_states=None):
+
+class DeviceBatch(_Batch):
+    """CSR batch on the GPU: pair p = (A[a_off[p]:a_off[p+1]], B[b_off[p]:b_off[p+1]]).
+
+    int32 pools (or ``signed=True``) select the int32 mode: sets sorted as signed integers."""
+
+    def __init__(self, algo, A, a_off, B, b_off, A_states=None, B_states=None, signed=None):
         torch = _torch()
         from .intersect import algo_id
 
@@ -105,12 +115,13 @@ class DeviceBatch(_Batch):
         self.dA_off = to_device_u64(a_off, torch)
         self.dB_off = to_device_u64(b_off, torch)
         self.bsr = A_states is not None
+        self.signed = _signed(signed, A, B)
         self.dAs = to_device_u32(A_states, torch) if self.bsr else None
         self.dBs = to_device_u32(B_states, torch) if self.bsr else None
         h = _c.c_void_p()
         _native.check(
             lib.setops_hip_batch_create(
-                _c.byref(h), algo_id(algo), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
+                _c.byref(h), algo_id(algo) | (_native.SIGNED_I32 if self.signed else 0), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
                 self.a_len, self.dA_off.data_ptr(), self.dB.data_ptr(), self.dBs.data_ptr() if self.bsr else None,
                 self.b_len, self.dB_off.data_ptr(), self.P,
             )
@@ -169,7 +180,7 @@ class DeviceBatch(_Batch):
         off = self.d_out_off.cpu().numpy().view(np.uint64).copy()
         cnt = self.d_counts.cpu().numpy().view(np.uint32).copy()
         total = int(off[-1])
-        data = self.d_out[:total].cpu().numpy().view(np.uint32).copy()
+        data = self.d_out[:total].cpu().numpy().view(np.int32 if self.signed else np.uint32).copy()
         if self.bsr:
             st = self.d_out_s[:total].cpu().numpy().view(np.uint32).copy()
             return off, cnt, data, st
@@ -200,7 +211,7 @@ class GatherBatch(DeviceBatch):
     the index arrays are numpy).  ``outputs=False`` allocates no result buffer (count-only use)."""
 
     def __init__(self, algo, A, a_beg, a_end, B, b_beg, b_end, A_states=None, B_states=None, capacity=None,
-                 outputs=True):
+                 outputs=True, signed=None):
         torch = _torch()
         from .intersect import algo_id
 
@@ -235,12 +246,13 @@ class GatherBatch(DeviceBatch):
         self.dB_beg = to_device_u64(b_beg, torch)
         self.dB_end = to_device_u64(b_end, torch)
         self.bsr = A_states is not None
+        self.signed = _signed(signed, A, B)
         self.dAs = to_device_u32(A_states, torch) if self.bsr else None
         self.dBs = to_device_u32(B_states, torch) if self.bsr else None
         h = _c.c_void_p()
         _native.check(
             lib.setops_hip_batch_create_gather(
-                _c.byref(h), algo_id(algo), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
+                _c.byref(h), algo_id(algo) | (_native.SIGNED_I32 if self.signed else 0), self.dA.data_ptr(), self.dAs.data_ptr() if self.bsr else None,
                 self.a_len, self.dA_beg.data_ptr(), self.dA_end.data_ptr(), self.dB.data_ptr(),
                 self.dBs.data_ptr() if self.bsr else None, self.b_len, self.dB_beg.data_ptr(),
                 self.dB_end.data_ptr(), self.P,

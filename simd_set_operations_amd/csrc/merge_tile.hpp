@@ -27,8 +27,8 @@ struct MergeSmem {
     // whole 16-byte chunks (<= 7 extra words per A and B window, +1 boundary element)
     static constexpr int KEYS = GATHER ? U + 16 * MAXSEG + 16 : U + 32;
     // + K + 8: the merge steps of a lane may read up to K words past its windows
-    uint32_t key_guard[4];  // read (never used) by a split search that starts at key[0]
-    alignas(16) uint32_t key[KEYS + K + 8];
+    alignas(16) uint32_t key_guard[4];  // key[-1] for the split search: read, never used
+    alignas(16) uint32_t key[KEYS + K + 8];  // starts 16 bytes after key_guard (checked below)
     alignas(16) uint32_t st[BSR ? KEYS : 4];
     static_assert(KEYS + K + 8 < 65536 && U < 65536, "segment fields are 16-bit LDS indices");
     // merge policy: per-segment result prefixes replace the per-lane masks (LDS is what limits how
@@ -51,6 +51,8 @@ struct MergeSmem {
     uint32_t win_c[GATHER ? 2 * MAXSEG + 1 : 1];
     uint64_t win_g[GATHER ? 2 * MAXSEG : 1];
     uint32_t win_src[GATHER ? 2 * MAXSEG : 1];
+    // byte offset of key[i] from key_guard is 4 * (i + 4): the split search addresses both from there
+    static constexpr uint32_t kGuardWords = 4;
 };
 
 // Lane policies: how a lane intersects its slice of the tile.
@@ -137,7 +139,7 @@ __device__ __forceinline__ uint32_t block_intersect(const uint32_t *key, const u
 template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP>
 __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR, GATHER, LP != LP_MERGE> &sm,
                                                    const TilePlan &plan, const TileOut &out, uint32_t t,
-                                                   uint32_t nseg, uint32_t ulen, uint32_t p_first)
+                                                   uint32_t nseg, uint32_t ulen, uint32_t p_first, uint32_t in_sflip)
 {
     constexpr bool BLOCKP = LP != LP_MERGE;
     using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
@@ -201,7 +203,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         } else {
         uint32_t kmax = 0;  // steps to run
         uint32_t pa = 0, pb = 0, ea = 0, eb = 0, ebp = 0, s = 0;
-        uint32_t q_lo = 0, q_n = 0, q_a4 = ~3u, q_s4 = ~3u;  // the lane's split search (q_n = 0: none)
+        // the lane's split search (q_n = 0: none; it then probes key[-1] against key[0])
+        uint32_t q_lo = 0, q_n = 0, q_a4 = 12u, q_s4 = 28u;
         bool q_on = false;
         lane_on = ub < ulen;
         if (ub < ulen) {
@@ -225,8 +228,10 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 pb = lb + e;
                 q_lo = (e > wb) ? e - wb : 0u;
                 q_n = ((e < wa) ? e : wa) - q_lo;
-                q_a4 = 4u * (la - 1u);
-                q_s4 = 4u * (la + lb + e - 1u);
+                // byte offsets from key_guard (key[i] at 4 * (i + 4)): A probe key[la + y - 1], B probe
+                // key[lb + e - y]; their sum is the lane constant q_s4
+                q_a4 = 4u * (la + 3u);
+                q_s4 = 4u * (la + lb + e + 7u);
                 q_on = true;
             }
             kmax = (ulen - ub < (uint32_t)K) ? ulen - ub : (uint32_t)K;
@@ -238,10 +243,12 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             // A[la + x'] <= B[lb + e - 1 - x'] for every x' < x (ties go to A), by binary lifting
             // with the wave's common step sequence (steps 2^(L-1) .. 1, L the wave's longest
             // range); a step that would pass the range end stops at it and changes nothing.
-            // Probe x = y - 1 reads A at byte 4(la + y - 1) and B at byte 4(lb + e - y), whose sum
-            // is the lane constant q_s4 (one guard word below key covers la + y - 1 = -1).
+            // Probe x = y - 1 reads A at byte 4(la + y - 1) and B at byte 4(lb + e - y) of key, both
+            // addressed from key_guard (16 bytes lower, so key[-1] = key_guard[3] is in range).
             const uint32_t L = wave_max_u32(32u - (uint32_t)__clz(q_n));
-            const char *kb = reinterpret_cast<const char *>(sm.key);
+            static_assert(offsetof(Smem, key) == offsetof(Smem, key_guard) + 4 * Smem::kGuardWords,
+                          "key follows key_guard directly");
+            const char *kb = reinterpret_cast<const char *>(sm.key_guard);
             const uint32_t qhi = q_lo + q_n;
             uint32_t x = q_lo;
             for (int bit = (int)L - 1; bit >= 0; bit--) {
@@ -298,7 +305,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 oa += takeA ? 4u : 0u;
                 asm volatile("" : "+v"(oa));
                 a = *reinterpret_cast<const uint32_t *>(kb + oa);
-                b = *reinterpret_cast<const uint32_t *>(kb + (s4 - oa) + 4u * (uint32_t)(k + 1));
+                // B's byte offset 4(pb + k + 1 - taken A) >= 4 pb: formed in 32-bit arithmetic first
+                b = *reinterpret_cast<const uint32_t *>(kb + ((s4 + 4u * (uint32_t)(k + 1)) - oa));
             }
             smask &= (kv >= 32u) ? ~0u : ((1u << kv) - 1u);
             const uint32_t rA = ea - pa;
@@ -358,9 +366,19 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         __syncthreads();
 
         // ---- 5. staged outputs (in order inside the tile's slot) ----------------------------------
+        // Sorted unique windows give at most (K + 1) / 2 results per lane, which the slot holds; a
+        // tile planned from sizes that disagree with the device offsets may not, and then writes
+        // nothing (error bit 1) rather than past its slot.
+        const uint32_t slot_keys = BSR ? plan.stage_words / 2 : plan.stage_words;
+        if (tot > slot_keys) {
+            if (tid == 0 && out.error) atomicOr(out.error, 2u);
+            tot = 0;
+            smask = 0;
+        }
         if constexpr (!COUNT_ONLY) {
             uint32_t *dk = plan.stage + (uint64_t)t * plan.stage_words;
             uint32_t *ds = dk + plan.stage_words / 2;
+            const uint32_t flip = in_sflip;  // LDS holds keys XOR flip; results are the keys as given
             const uint32_t kt_mask = (kt >= 32u) ? ~0u : ((1u << kt) - 1u);
             auto a_index = [&](uint32_t k) -> uint32_t {
                 const uint32_t before = tmask & ((1u << k) - 1u);
@@ -374,7 +392,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                     sm_ &= sm_ - 1;
                     const uint32_t ai = (x < nfirst) ? pa0 + x : pa1 + (x - nfirst);
                     const uint32_t v = sm.key[ai];
-                    dk[pos] = v;
+                    dk[pos] = v ^ flip;
                     if constexpr (BSR) {
                         // the matching B base lies in the same sub-window: find it again
                         const uint32_t jb = (x < nfirst) ? pb0 : pb1, nbw = (x < nfirst) ? nb0 : nb1;
@@ -402,7 +420,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                 while (sm_) {
                     const uint32_t k = __builtin_ctz(sm_);
                     sm_ &= sm_ - 1;
-                    dk[pos++] = sm.key[a_index(k)];
+                    dk[pos++] = sm.key[a_index(k)] ^ flip;
                 }
             }
         }
@@ -473,9 +491,11 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
         uint32_t nseg = p_last - p_first + 1;
-        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG;
+        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
+        if (t == 0 && tid == 0 && out.error && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
         SegWin w{};
+        uint32_t staged_words = 0;  // CSR: words loaded straight into LDS (flipped in place for int32 sets)
 
         // ---- 2. stage the windows --------------------------------------------------------------
         if constexpr (!GATHER) {
@@ -494,6 +514,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
             const uint32_t cA = (shift_a + n_a + 3) / 4, cB = (shift_b + n_b + 3) / 4;
             const uint32_t boff = 4u * cA;
+            staged_words = 4u * (cA + cB);
             // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks),
             // loaded straight into LDS; the segment table's loads are in flight meanwhile
             if (!bad) {
@@ -542,10 +563,10 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
             if (!bad) {
                 stage_windows<NT, QWIN>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, 2 * nseg,
-                                        sm.key);
+                                        sm.key, in.sflip);
                 if constexpr (BSR)
                     stage_windows<NT, QWIN>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src,
-                                            2 * nseg, sm.st);
+                                            2 * nseg, sm.st, 0u);
             }
         }
         if (bad) {
@@ -564,8 +585,16 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         if (tid == 0) sm.seg_u[nseg] = ulen;
         if constexpr (!GATHER) __builtin_amdgcn_s_waitcnt(0);  // the direct-to-LDS loads have landed
         __syncthreads();
+        if constexpr (!GATHER && !BSR) {
+            // int32 sets: flip bit 31 of the staged words, so unsigned compares give signed order
+            if (in.sflip) {
+                for (uint32_t i = tid; i < staged_words; i += NT) sm.key[i] ^= in.sflip;
+                __syncthreads();
+            }
+        }
 
-        merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first);
+        merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first,
+                                                                       in.sflip);
     }
 }
 

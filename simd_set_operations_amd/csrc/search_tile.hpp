@@ -128,13 +128,14 @@ __device__ __forceinline__ uint32_t lower_bound_block(const Arr &arr, uint32_t l
 
 // lower_bound_block over a window of a resident pool, L = P + g, P readable up to pool elements
 // and 16-byte aligned: the last block is read as the aligned 16-byte chunks that cover it (one
-// round trip).  Also reports whether the window holds x.
+// round trip).  Also reports whether the window holds x.  Pool values are compared XOR f (x is
+// already in that space).
 __device__ __forceinline__ uint32_t lower_bound_block_pool(const uint32_t *__restrict__ P, uint64_t pool, uint64_t g,
-                                                           uint32_t lo, uint32_t hi, uint32_t x, bool &found)
+                                                           uint32_t lo, uint32_t hi, uint32_t x, uint32_t f, bool &found)
 {
     constexpr uint32_t W = kBlockPool;
     const uint32_t *__restrict__ L = P + g;
-    auto arr = [&](uint32_t i) { return L[i]; };
+    auto arr = [&](uint32_t i) { return L[i] ^ f; };
     const uint32_t b = gallop_blocks<W>(arr, lo, hi, x);
     const uint32_t s0 = lo + W * b;
     found = false;
@@ -148,7 +149,7 @@ __device__ __forceinline__ uint32_t lower_bound_block_pool(const uint32_t *__res
         const uint64_t e = c0 + 4ull * (uint64_t)c;
         if (e < q1) {
             const uint4 v = load_chunk(P, pool, e);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            const uint32_t w[4] = {v.x ^ f, v.y ^ f, v.z ^ f, v.w ^ f};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const uint64_t q = e + (uint64_t)j;
@@ -164,12 +165,13 @@ __device__ __forceinline__ uint32_t lower_bound_block_pool(const uint32_t *__res
 
 constexpr uint64_t kNoPos = ~0ull;
 
-// Pool index of x in B[g0, g1), or kNoPos, using idx[k] = B[32k] (every 32nd pool element): a
-// binary search over the index entries inside the range finds the only 32-element line that can
-// hold x, which is compared in full (eight 16-byte loads).  The range is sorted (one set), so the
-// index entries inside it are too.
+// Pool index of x in B[g0, g1), or kNoPos, using idx[k] = B[32k] ^ f (every 32nd pool element, in
+// the compare space): a binary search over the index entries inside the range finds the only
+// 32-element line that can hold x, which is compared in full (eight 16-byte loads).  The range is
+// sorted (one set), so the index entries inside it are too.
 __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, uint64_t pool,
-                                                const uint32_t *__restrict__ idx, uint64_t g0, uint64_t g1, uint32_t x)
+                                                const uint32_t *__restrict__ idx, uint64_t g0, uint64_t g1, uint32_t x,
+                                                uint32_t f)
 {
     if (g0 >= g1) return kNoPos;
     const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;  // entries with 32k in [g0, g1)
@@ -185,7 +187,7 @@ __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, 
     for (int c = 0; c < 8; c++) {
         const uint64_t e = base + 4ull * c;
         const uint4 v = load_chunk(B, pool, e);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        const uint32_t w[4] = {v.x ^ f, v.y ^ f, v.z ^ f, v.w ^ f};
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint64_t q = e + (uint64_t)j;
@@ -226,7 +228,7 @@ __device__ __forceinline__ uint32_t count_le(const uint32_t *__restrict__ arr, u
 __device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__ B, uint64_t pool,
                                                     const uint32_t *__restrict__ idx,
                                                     const uint32_t *__restrict__ mid, const uint32_t *top,
-                                                    uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x)
+                                                    uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x, uint32_t f)
 {
     if (g0 >= g1) return kNoPos;
     const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;                // entries with 32k in [g0, g1)
@@ -253,7 +255,7 @@ __device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__
     for (int c = 0; c < 8; c++) {
         const uint64_t e = base + 4ull * c;
         const uint4 v = load_chunk(B, pool, e);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        const uint32_t w[4] = {v.x ^ f, v.y ^ f, v.z ^ f, v.w ^ f};
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint64_t q = e + (uint64_t)j;
@@ -284,8 +286,9 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
         uint32_t nseg = p_last - p_first + 1;
-        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG;
+        bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
+        if (t == 0 && tid == 0 && out.error && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
         uint32_t cp = 0;
         uint64_t gp = 0;
         uint32_t swap = 0;
@@ -321,9 +324,11 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             if (tid == 0 && out.error) atomicOr(out.error, 2u);
         } else {
             constexpr int QP = (Smem::PKEYS + 4 * NT - 1) / (4 * NT);
-            stage_windows<NT, QP>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey);
+            stage_windows<NT, QP>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey,
+                                  in.sflip);
             if constexpr (BSR)
-                stage_windows<NT, QP>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst);
+                stage_windows<NT, QP>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst,
+                                      0u);
         }
         __syncthreads();
 
@@ -380,8 +385,9 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 const bool large_is_a = (sm.seg_flags[s] >> 2) & 1u;
                 const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
                 const uint32_t vf = sm.pkey[sm.seg_la[s]], vl = sm.pkey[sm.seg_la[s] + n - 1];
-                lo = (uint32_t)group_partition_point<kWave>(0, nl, lane, 0, [&](uint64_t x) { return L[x] < vf; });
-                hi = (uint32_t)group_partition_point<kWave>(lo, nl, lane, 0, [&](uint64_t x) { return L[x] <= vl; });
+                const uint32_t f = in.sflip;
+                lo = (uint32_t)group_partition_point<kWave>(0, nl, lane, 0, [&](uint64_t x) { return (L[x] ^ f) < vf; });
+                hi = (uint32_t)group_partition_point<kWave>(lo, nl, lane, 0, [&](uint64_t x) { return (L[x] ^ f) <= vl; });
             }
             if (lane == 0) {
                 sm.seg_lo[s] = lo;
@@ -427,10 +433,11 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             __syncthreads();
             if (nst) {
                 constexpr int QL = (Smem::LKEYS + 4 * NT - 1) / (4 * NT);
-                stage_windows<NT, QL>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst, sm.lkey);
+                stage_windows<NT, QL>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst, sm.lkey,
+                                      in.sflip);
                 if constexpr (BSR)
                     stage_windows<NT, QL>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nst,
-                                          sm.lst);
+                                          sm.lst, 0u);
             }
             __syncthreads();
         }
@@ -446,10 +453,11 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             for (uint32_t j = wave; j < nx; j += NW) {
                 const uint32_t *__restrict__ X = in.xsets[j];
                 const uint64_t nxj = in.xlens[j];
-                const uint32_t lo =
-                    (uint32_t)group_partition_point<kWave>(0, nxj, lane, 0, [&](uint64_t i) { return X[i] < vf; });
-                const uint32_t hi =
-                    (uint32_t)group_partition_point<kWave>(lo, nxj, lane, 0, [&](uint64_t i) { return X[i] <= vl; });
+                const uint32_t f = in.sflip;
+                const uint32_t lo = (uint32_t)group_partition_point<kWave>(0, nxj, lane, 0,
+                                                                           [&](uint64_t i) { return (X[i] ^ f) < vf; });
+                const uint32_t hi = (uint32_t)group_partition_point<kWave>(
+                    lo, nxj, lane, 0, [&](uint64_t i) { return (X[i] ^ f) <= vl; });
                 if (lane == 0) {
                     sm.xlo[j] = lo;
                     sm.xhi[j] = hi;
@@ -500,9 +508,9 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     // resident B pool with an index of every 32nd element: binary search the index,
                     // then compare one 128-byte line of B
                     const uint64_t g = sm.seg_lbase[s];
-                    const uint64_t q = top_ok
-                        ? index_probe_top(in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g + cur, g + hi_s, x)
-                        : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x);
+                    const uint64_t q = top_ok ? index_probe_top(in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g + cur,
+                                                                g + hi_s, x, in.sflip)
+                                              : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x, in.sflip);
                     found = q != kNoPos;
                     pos = found ? (uint32_t)(q - g) : cur;
                     if constexpr (BSR)
@@ -510,12 +518,13 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 } else {
                     if constexpr (SMODE == SM_BLOCK) {
                         pos = lower_bound_block_pool(large_is_a ? in.A : in.B, large_is_a ? in.a_pool : in.b_pool,
-                                                     sm.seg_lbase[s], cur, hi_s, x, found);
+                                                     sm.seg_lbase[s], cur, hi_s, x, in.sflip, found);
                     } else {
                         const uint32_t *__restrict__ L = (large_is_a ? in.A : in.B) + sm.seg_lbase[s];
-                        auto arr = [&](uint32_t i) { return L[i]; };
+                        const uint32_t f = in.sflip;
+                        auto arr = [&](uint32_t i) { return L[i] ^ f; };
                         pos = lower_bound_from<SMODE == SM_GALLOP>(arr, cur, hi_s, x);
-                        found = pos < hi_s && L[pos] == x;
+                        found = pos < hi_s && arr(pos) == x;
                     }
                     if constexpr (BSR)
                         if (found) lstate = (large_is_a ? in.As : in.Bs)[sm.seg_lbase[s] + pos];
@@ -525,11 +534,12 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                     // further sets in order; the first one that lacks x ends the lookup
                     for (uint32_t j = 0; found && j < nx; j++) {
                         const uint32_t *__restrict__ X = in.xsets[j];
-                        auto xarr = [&](uint32_t i) { return X[i]; };
+                        const uint32_t f = in.sflip;
+                        auto xarr = [&](uint32_t i) { return X[i] ^ f; };
                         const uint32_t xh = sm.xhi[j];
                         const uint32_t xp = lower_bound_from<SMODE != SM_BINARY>(xarr, xcur[j], xh, x);
                         xcur[j] = xp;
-                        found = xp < xh && X[xp] == x;
+                        found = xp < xh && xarr(xp) == x;
                     }
                 }
                 if constexpr (BSR) {
@@ -565,7 +575,7 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 const uint32_t u = ub + (uint32_t)k;
                 if (hmask & (1u << k)) {
                     while (u >= sm.seg_u[s + 1]) s++;
-                    dk[pos] = sm.pkey[sm.seg_la[s] + (u - sm.seg_u[s])];
+                    dk[pos] = sm.pkey[sm.seg_la[s] + (u - sm.seg_u[s])] ^ in.sflip;  // the value as given
                     if constexpr (BSR) ds[pos] = svals[k];
                     pos++;
                 }

@@ -161,10 +161,23 @@ int ensure_init(int device)
         if (rc_ != SETOPS_OK) return rc_;      \
     } while (0)
 
+// Registered entry of an algorithm id; SETOPS_SIGNED_I32 may be OR'ed into the id.
 const AlgoEntry *algo_entry(int algo)
 {
+    algo &= ~SETOPS_SIGNED_I32;
     if (algo < 0 || algo >= kNumAlgos) return nullptr;
     return &kAlgos[algo];
+}
+int algo_base(int algo) { return algo & ~SETOPS_SIGNED_I32; }
+// PairBatch::sflip of an algorithm id: int32 sets compare with bit 31 flipped.
+uint32_t sflip_of(int algo) { return (algo & SETOPS_SIGNED_I32) ? 0x80000000u : 0u; }
+// int32 mode exists for plain entries only: the reference's BSR form is built from u32 values
+// (benchmark/src/timer/harness.rs:121-122 casts the i32 sets to u32 before BsrVec::from_sorted).
+int check_signed(int algo, const AlgoEntry *e)
+{
+    if ((algo & SETOPS_SIGNED_I32) && e->bsr)
+        return fail(SETOPS_ERR_UNSUPPORTED, "%s: BSR entries take u32 bases (no int32 mode)", e->name);
+    return SETOPS_OK;
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -257,7 +270,9 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
     return SETOPS_OK;
 }
 
-int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
+// Lays out the workspace of P pairs and ntiles tiles at `b` and points the plan into it (b == NULL:
+// sizes only).  Returns its bytes.
+size_t ws_layout(setops_hip_batch *h, uint64_t P, uint64_t ntiles, char *b)
 {
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -281,14 +296,7 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
     const size_t o_ge = take(ngroups * 8);
     const size_t o_pl = take(P * 4 + 4);
     const size_t o_misc = take(256);  // error word, scratch totals, single-pair offsets
-    void *ws = nullptr;
-    if (hipMalloc(&ws, off) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %zu workspace bytes failed", off);
-    }
-    char *b = static_cast<char *>(ws);
-    h->ws = ws;
-    h->ws_bytes = off;
+    if (!b) return off;
     h->plan.uoff = reinterpret_cast<uint64_t *>(b + o_uoff);
     h->plan.tile_first = reinterpret_cast<uint32_t *>(b + o_tf);
     h->plan.tile_last = reinterpret_cast<uint32_t *>(b + o_tl);
@@ -306,6 +314,20 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
     h->d_tot = reinterpret_cast<unsigned long long *>(b + o_misc + 16);
     h->plan.single_off = reinterpret_cast<uint64_t *>(b + o_misc + 64);
     h->plan.stage_words = sw;
+    return off;
+}
+
+int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
+{
+    const size_t bytes = ws_layout(h, P, ntiles, nullptr);
+    void *ws = nullptr;
+    if (hipMalloc(&ws, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SETOPS_ERR_NOMEM, "hipMalloc of %zu workspace bytes failed", bytes);
+    }
+    h->ws = ws;
+    h->ws_bytes = bytes;
+    ws_layout(h, P, ntiles, static_cast<char *>(ws));
     return SETOPS_OK;
 }
 
@@ -368,6 +390,9 @@ int device_unit_total(setops_hip_batch *h, uint64_t *utotal)
 {
     const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
     hipStream_t s = g_ctx.stream;
+    // the offsets may have been written by work still pending on another stream (this count runs
+    // on the library's own stream, and creation synchronises anyway)
+    HIP_TRY(hipDeviceSynchronize());
     hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, s, h->in, h->plan);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, s, h->plan.gsum, h->plan.gsum2,
@@ -516,6 +541,8 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.counts = d_counts;
     out.totals = d_totals;
     out.error = h->d_error;
+    // a run on another stream than the last one would share the workspace and staging with it
+    if (h->last_stream && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
     if (!count_only) {
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;
@@ -584,6 +611,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     *handle = nullptr;
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
+    if (int rc = check_signed(algo, e)) return rc;
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
     const bool gather = d_a_end != nullptr;
     if (!d_a_beg || !d_b_beg || (gather && !d_b_end)) return fail(SETOPS_ERR_INVALID, "offset arrays must not be NULL");
@@ -596,10 +624,11 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
         return fail(SETOPS_ERR_INVALID, "state arrays given to a plain algorithm");
     }
     auto *h = new setops_hip_batch();
-    h->algo = algo;
+    h->algo = algo_base(algo);
     h->kind = e->kind;
     h->bsr = e->bsr;
     h->gather = gather;
+    h->in.sflip = sflip_of(algo);
     h->P = npairs;
     h->in.A = d_a;
     h->in.As = d_as;
@@ -684,13 +713,14 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
                   hipStream_t s, const uint32_t *const *dX = nullptr, const uint64_t *dXlen = nullptr, uint32_t nx = 0)
 {
     const AlgoEntry *e = algo_entry(algo);
-    setops_hip_batch *&h = g_ctx.single[algo];
+    setops_hip_batch *&h = g_ctx.single[algo_base(algo)];
     if (!h) {
         h = new setops_hip_batch();
-        h->algo = algo;
+        h->algo = algo_base(algo);
         h->kind = e->kind;
         h->bsr = e->bsr;
     }
+    h->in.sflip = sflip_of(algo);
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
     const uint64_t units =
@@ -896,6 +926,7 @@ int setops_hip_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_TWOSET)) return fail(SETOPS_ERR_UNSUPPORTED, "%s is not a plain 2-set entry", e->name);
+    if (int rc = check_signed(algo, e)) return rc;
     if ((na && !a) || (nb && !b)) return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if (!out_len) return fail(SETOPS_ERR_INVALID, "out_len is NULL");
     ENSURE_INIT();
@@ -903,12 +934,15 @@ int setops_hip_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t
     return run_single_pair(algo, a, nullptr, na, b, nullptr, nb, out, nullptr, out_cap, out_len, nullptr);
 }
 
-int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na, const uint32_t *d_b,
-                              const uint32_t *d_b_states, size_t nb, uint32_t *d_out, uint32_t *d_out_states,
-                              size_t out_cap, unsigned long long *d_totals, void *stream)
+namespace {
+// Argument checks shared by the device-pointer pair calls.
+int check_dev_pair(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na, const uint32_t *d_b,
+                   const uint32_t *d_b_states, size_t nb, uint32_t *d_out, uint32_t *d_out_states,
+                   unsigned long long *d_totals)
 {
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
+    if (int rc = check_signed(algo, e)) return rc;
     if ((na && !d_a) || (nb && !d_b)) return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if (!aligned16(d_a) || !aligned16(d_b)) return fail(SETOPS_ERR_INVALID, "inputs must be 16-byte aligned");
     if (!d_totals) return fail(SETOPS_ERR_INVALID, "d_totals is NULL");
@@ -921,6 +955,16 @@ int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a
     } else if (d_a_states || d_b_states || d_out_states) {
         return fail(SETOPS_ERR_INVALID, "state arrays given to a plain entry");
     }
+    return SETOPS_OK;
+}
+} // namespace
+
+int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na, const uint32_t *d_b,
+                              const uint32_t *d_b_states, size_t nb, uint32_t *d_out, uint32_t *d_out_states,
+                              size_t out_cap, unsigned long long *d_totals, void *stream)
+{
+    if (int rc = check_dev_pair(algo, d_a, d_a_states, na, d_b, d_b_states, nb, d_out, d_out_states, d_totals))
+        return rc;
     ENSURE_INIT();
     std::lock_guard<std::mutex> lk(g_ctx.host_mu);
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL = the HIP null stream
@@ -934,6 +978,7 @@ int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t 
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_BSR)) return fail(SETOPS_ERR_UNSUPPORTED, "%s is not a BSR entry", e->name);
+    if (int rc = check_signed(algo, e)) return rc;
     if ((na && (!bases_a || !states_a)) || (nb && (!bases_b || !states_b)))
         return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if ((bases_out == nullptr) != (states_out == nullptr))
@@ -951,6 +996,7 @@ int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, si
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_KSET)) return fail(SETOPS_ERR_UNSUPPORTED, "%s cannot drive svs_generic", e->name);
+    if (int rc = check_signed(algo, e)) return rc;
     if (k < 2) return fail(SETOPS_ERR_INVALID, "svs needs at least 2 sets (svs.rs:61)");
     if (!sets || !lens || !out_len) return fail(SETOPS_ERR_INVALID, "NULL argument");
     for (size_t s = 0; s < k; s++)
@@ -979,6 +1025,21 @@ int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, si
         std::memcpy(out, left.data(), n * 4);
     }
     return SETOPS_OK;
+}
+
+int setops_hip_intersect2_i32(int algo, const int32_t *a, size_t na, const int32_t *b, size_t nb, int32_t *out,
+                              size_t out_cap, size_t *out_len)
+{
+    return setops_hip_intersect2(algo | SETOPS_SIGNED_I32, reinterpret_cast<const uint32_t *>(a), na,
+                                 reinterpret_cast<const uint32_t *>(b), nb, reinterpret_cast<uint32_t *>(out), out_cap,
+                                 out_len);
+}
+
+int setops_hip_svs_i32(int algo, const int32_t *const *sets, const size_t *lens, size_t k, int32_t *out, size_t out_cap,
+                       size_t *out_len)
+{
+    return setops_hip_svs(algo | SETOPS_SIGNED_I32, reinterpret_cast<const uint32_t *const *>(sets), lens, k,
+                          reinterpret_cast<uint32_t *>(out), out_cap, out_len);
 }
 
 int setops_hip_batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const uint32_t *d_a_states,
@@ -1024,15 +1085,19 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
         handle->b_mid = nullptr;
         return fail(SETOPS_ERR_NOMEM, "hipMalloc of %llu index bytes failed", (unsigned long long)((mt + mm) * 4));
     }
+    // the B pool may have been written by work still pending on another stream; the index build
+    // waits for it (it synchronises at the end anyway)
+    HIP_TRY(hipDeviceSynchronize());
     hipStream_t s = handle->last_stream ? handle->last_stream : g_ctx.stream;
+    const uint32_t f = handle->in.sflip;  // the index holds values in the compare space
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
-                       handle->b_idx, (uint64_t)32);
+                       handle->b_idx, (uint64_t)32, f);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(mt, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
-                       handle->b_top, kTopStride);
+                       handle->b_top, kTopStride, f);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(mm, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
-                       handle->b_mid, kMidStride);
+                       handle->b_mid, kMidStride, f);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     handle->in.b_idx = handle->b_idx;
@@ -1083,6 +1148,9 @@ int setops_hip_batch_check(setops_hip_batch *handle)
     uint32_t err = 0;
     HIP_TRY(hipMemcpyAsync(&err, handle->d_error, 4, hipMemcpyDeviceToHost, handle->last_stream));
     HIP_TRY(hipStreamSynchronize(handle->last_stream));
+    if (err & 4u)
+        return fail(SETOPS_ERR_INVALID, "the pair sizes on the device differ from the plan (setops_hip_batch_rebind "
+                                         "sizes must equal the device offsets); results are incomplete");
     if (err & 2u) return fail(SETOPS_ERR_INVALID, "internal error: a tile exceeded its segment bound");
     if (err & 1u) return fail(SETOPS_ERR_CAPACITY, "batch output exceeded its capacity (writes were clipped)");
     return SETOPS_OK;

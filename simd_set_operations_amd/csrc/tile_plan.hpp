@@ -41,6 +41,9 @@ struct PairBatch {
     uint64_t b_pool;
     uint64_t P;
     uint32_t contig;  // 1: a_end[p] == a_beg[p+1] and b_end[p] == b_beg[p+1] for every p (CSR)
+    // 0x80000000 for sorted int32 sets (plain sets only), else 0.  Every value comparison is made
+    // on values XOR sflip, where unsigned order is signed order; values are written out unchanged.
+    uint32_t sflip;
     const uint32_t *b_idx;  // optional: b_idx[k] = B[32k] (search policy: resident B pools)
     const uint32_t *b_top;  // optional, with b_idx: b_top[j] = B[kTopStride j] (staged in LDS per tile)
     const uint32_t *b_mid;  // with b_top: b_mid[m] = B[kMidStride m]
@@ -59,13 +62,13 @@ constexpr uint64_t kMidIdx = 16;               // index entries per middle-level
 constexpr uint64_t kMidStride = 32 * kMidIdx;  // pool elements per middle-level entry (512)
 static_assert(kTopIdx / kMidIdx <= 16 && kMidIdx <= 16, "a level spans at most 16 entries of the next");
 
-// Index of a resident pool: idx[k] = pool[stride * k], k < ceil(n / stride).
+// Index of a resident pool in the compare space: idx[k] = pool[stride * k] ^ flip, k < ceil(n / stride).
 __global__ void build_index_kernel(const uint32_t *__restrict__ pool, uint64_t n, uint32_t *__restrict__ idx,
-                                   uint64_t stride)
+                                   uint64_t stride, uint32_t flip)
 {
     const uint64_t m = (n + stride - 1) / stride;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
-        idx[k] = pool[stride * k];
+        idx[k] = pool[stride * k] ^ flip;
 }
 
 enum : uint32_t { POLICY_MERGE = 0, POLICY_SEARCH = 1 };
@@ -108,7 +111,8 @@ struct TileOut {
     uint64_t *out_off;      // [P+1] output offset of each pair (may be null)
     uint32_t *counts;       // [P] per-pair output count (may be null)
     unsigned long long *totals;  // [2]: output words, output elements (popcount for BSR)
-    uint32_t *error;        // bit 0: output capacity exceeded, bit 1: tile bound violated
+    uint32_t *error;        // bit 0: output capacity exceeded, bit 1: tile bound violated,
+                            // bit 2: the device pair sizes disagree with the plan's unit total
 };
 
 __device__ __forceinline__ uint64_t pair_units(const PairBatch &in, uint32_t policy, uint32_t c0, uint64_t p)
@@ -295,6 +299,11 @@ __global__ void plan_split_kernel(PairBatch in, TilePlan plan)
     const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < plan.ntiles; t += gsz) {
         const uint64_t p = plan.tile_first[t];
+        if (p >= in.P) {  // a tile no pair reached (device sizes differ from the plan): flagged by the tile kernel
+            plan.tile_sa[t] = 0;
+            plan.tile_sb[t] = 0;
+            continue;
+        }
         const uint64_t a = in.a_beg[p], b = in.b_beg[p];
         const uint64_t na = in.a_end[p] - a, nb = in.b_end[p] - b;
         const uint64_t rel = (uint64_t)t * plan.U - plan.uoff[p];
@@ -308,9 +317,10 @@ __global__ void plan_split_kernel(PairBatch in, TilePlan plan)
         uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
         const uint32_t *__restrict__ A = in.A + a;
         const uint32_t *__restrict__ B = in.B + b;
+        const uint32_t f = in.sflip;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
-            if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
+            if ((A[mid] ^ f) <= (B[d - 1 - mid] ^ f)) lo = mid + 1; else hi = mid;
         }
         plan.tile_sa[t] = (uint32_t)lo;
         plan.tile_sb[t] = (uint32_t)(d - lo);
@@ -517,7 +527,9 @@ __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan 
             // group's end: all ones, so the search never stops there
             if (tid <= TPW) toff[tid] = (tid <= ntw) ? (uint32_t)(plan.tile_off[tw0 + tid] - base) : ~0u;
         }
-        const uint64_t pf = plan.tile_first[tw0], pl = plan.tile_last[tw0 + ntw - 1];
+        const uint64_t pf = plan.tile_first[tw0];
+        uint64_t pl = plan.tile_last[tw0 + ntw - 1];
+        if (pl >= in.P) pl = in.P - 1;  // stale maps (flagged by the tile kernel): stay inside the pairs
         // the first pairs' prefix entries and in-tile offsets load with the tile offsets
         const uint64_t p = pf + tid;
         const bool pv = out.out_off && p <= pl;
@@ -692,12 +704,12 @@ __device__ __forceinline__ void stage_run_lds(const uint32_t *__restrict__ src, 
 // dst + 4*cpfx[w] + (g0_w & 3) as whole aligned 16-byte chunks.  cpfx is the exclusive prefix of
 // the windows' chunk counts (cpfx[nwin] = total).  Each thread takes chunks c, c + NT, ... (at most
 // QMAX of them), finds each chunk's window by binary search over cpfx, and issues all its loads
-// before storing.
+// before storing.  Every staged word is XORed with `flip` (PairBatch::sflip for keys, 0 for states).
 template <int NT, int QMAX>
 __device__ __forceinline__ void stage_windows(const uint32_t *__restrict__ srcA, uint64_t poolA,
                                               const uint32_t *__restrict__ srcB, uint64_t poolB,
                                               const uint32_t *cpfx, const uint64_t *gbase, const uint32_t *wsrc,
-                                              uint32_t nwin, uint32_t *dst)
+                                              uint32_t nwin, uint32_t *dst, uint32_t flip)
 {
     const uint32_t total = cpfx[nwin];
     uint4 v[QMAX];
@@ -717,7 +729,14 @@ __device__ __forceinline__ void stage_windows(const uint32_t *__restrict__ srcA,
 #pragma unroll
     for (int k = 0; k < QMAX; k++) {
         const uint32_t c = threadIdx.x + (uint32_t)k * NT;
-        if (c < total) *reinterpret_cast<uint4 *>(dst + 4 * c) = v[k];
+        if (c < total) {
+            uint4 x = v[k];
+            x.x ^= flip;
+            x.y ^= flip;
+            x.z ^= flip;
+            x.w ^= flip;
+            *reinterpret_cast<uint4 *>(dst + 4 * c) = x;
+        }
     }
 }
 

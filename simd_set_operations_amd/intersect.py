@@ -29,6 +29,35 @@ def _u32(a):
     return arr
 
 
+def _is_int32(*sets):
+    return any(np.asarray(x).dtype == np.int32 for x in sets)
+
+
+def _keys(*sets):
+    """The sets as u32 bit patterns for the C ABI, and whether they are int32 sets.
+
+    An int32 input selects the int32 mode (SETOPS_SIGNED_I32: signed order, the element type of
+    the reference's harness, benchmark/src/timer.rs:19); every set must then fit int32.  Otherwise
+    the sets must fit u32."""
+    signed = _is_int32(*sets)
+    out = []
+    for x in sets:
+        arr = np.ascontiguousarray(x)
+        if signed:
+            if arr.dtype != np.int32:
+                if arr.size and (arr.min() < -(1 << 31) or arr.max() > (1 << 31) - 1):
+                    raise ValueError("int32 sets: every element must fit in int32")
+                arr = arr.astype(np.int32)
+            out.append(arr.view(np.uint32))
+        else:
+            out.append(_u32(arr))
+    return out, signed
+
+
+def _result(arr, signed):
+    return arr.view(np.int32) if signed else arr
+
+
 def _ptr(arr):
     return arr.ctypes.data_as(_c.c_void_p) if arr.size else None
 
@@ -49,17 +78,20 @@ def registered():
 
 def intersect2_c(name, set_a, set_b, out=None):
     """Intersect2C (intersect.rs:41): writes into ``out`` (capacity >= result) and returns the
-    count.  ``out=None`` is the count-only call."""
+    count.  ``out=None`` is the count-only call.  int32 sets (and an int32 ``out``) use the int32
+    mode."""
     lib = _native.hip()
-    a, b = _u32(set_a), _u32(set_b)
+    (a, b), signed = _keys(set_a, set_b)
+    algo = algo_id(name) | (_native.SIGNED_I32 if signed else 0)
     n = _c.c_size_t(0)
     if out is None:
-        rc = lib.setops_hip_intersect2(algo_id(name), _ptr(a), a.size, _ptr(b), b.size, None, 0, _c.byref(n))
+        rc = lib.setops_hip_intersect2(algo, _ptr(a), a.size, _ptr(b), b.size, None, 0, _c.byref(n))
     else:
-        if not (isinstance(out, np.ndarray) and out.dtype == np.uint32 and out.flags.c_contiguous):
-            raise TypeError("out must be a contiguous uint32 array")
+        want = np.int32 if signed else np.uint32
+        if not (isinstance(out, np.ndarray) and out.dtype == want and out.flags.c_contiguous):
+            raise TypeError(f"out must be a contiguous {np.dtype(want).name} array")
         rc = lib.setops_hip_intersect2(
-            algo_id(name), _ptr(a), a.size, _ptr(b), b.size, _ptr(out) if out.size else None, out.size, _c.byref(n)
+            algo, _ptr(a), a.size, _ptr(b), b.size, _ptr(out) if out.size else None, out.size, _c.byref(n)
         )
     _native.check(rc)
     return n.value
@@ -97,7 +129,7 @@ def _make_twoset(name):
             visitor.add(intersect2_c(name, set_a, set_b, None))
             return
         cap = min(len(set_a), len(set_b))
-        out = np.zeros(max(cap, 1), np.uint32)
+        out = np.zeros(max(cap, 1), np.int32 if _is_int32(set_a, set_b) else np.uint32)
         n = intersect2_c(name, set_a, set_b, out)
         visitor.visit_many(out[:n])
 
@@ -160,7 +192,7 @@ def run_2set(set_a, set_b, intersect):
 def run_2set_c(set_a, set_b, intersect):
     """intersect.rs:54-68: allocate min(|a|,|b|), call the Intersect2C entry, truncate."""
     name = intersect if isinstance(intersect, str) else intersect.setops_name
-    out = np.zeros(max(min(len(set_a), len(set_b)), 1), np.uint32)
+    out = np.zeros(max(min(len(set_a), len(set_b)), 1), np.int32 if _is_int32(set_a, set_b) else np.uint32)
     n = intersect2_c(name, set_a, set_b, out)
     return out[:n].copy()
 
@@ -198,7 +230,7 @@ def run_kset(sets, intersect):
         raise ValueError("k-set intersection needs at least 2 sets")
     name = intersect if isinstance(intersect, str) else intersect.setops_name
     lib = _native.hip()
-    arrs = [_u32(s) for s in sets]
+    arrs, signed = _keys(*sets)
     k = len(arrs)
     ptrs = (_c.c_void_p * k)(*[a.ctypes.data if a.size else None for a in arrs])
     lens = (_c.c_size_t * k)(*[a.size for a in arrs])
@@ -206,7 +238,7 @@ def run_kset(sets, intersect):
     out = np.zeros(max(cap, 1), np.uint32)
     n = _c.c_size_t(0)
     _native.check(
-        lib.setops_hip_svs(algo_id(name), _c.cast(ptrs, _c.c_void_p), _c.cast(lens, _c.c_void_p), k, _ptr(out),
-                           cap, _c.byref(n))
+        lib.setops_hip_svs(algo_id(name) | (_native.SIGNED_I32 if signed else 0), _c.cast(ptrs, _c.c_void_p),
+                           _c.cast(lens, _c.c_void_p), k, _ptr(out), cap, _c.byref(n))
     )
-    return out[: n.value].copy()
+    return _result(out[: n.value].copy(), signed)

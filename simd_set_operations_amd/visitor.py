@@ -57,18 +57,26 @@ class Counter(Visitor, Clearable):
 
 
 class VecWriter(Visitor, Clearable):
-    """Appends visits to a growable vector (visitor.rs:51-97)."""
+    """Appends visits to a growable vector (visitor.rs:51-97).  Like VecWriter<T>, it holds the
+    element type it is given: int32 visits (the int32 mode) stay int32, anything else is u32."""
 
-    def __init__(self, capacity=0):
+    def __init__(self, capacity=0, dtype=None):
         self._parts = []
         self._n = 0
+        self._dtype = dtype
+
+    def _as(self, values):
+        v = np.asarray(values)
+        if self._dtype is None:
+            self._dtype = np.int32 if v.dtype == np.int32 else np.uint32
+        return v.astype(self._dtype, copy=False)
 
     def visit(self, value):
-        self._parts.append(np.asarray([value], dtype=np.uint32))
+        self._parts.append(self._as(np.asarray([value])))
         self._n += 1
 
     def visit_many(self, values):
-        v = np.asarray(values, dtype=np.uint32)
+        v = self._as(values)
         if v.size:
             self._parts.append(v.copy())
             self._n += v.size
@@ -79,7 +87,7 @@ class VecWriter(Visitor, Clearable):
 
     def as_array(self):
         if not self._parts:
-            return np.zeros(0, dtype=np.uint32)
+            return np.zeros(0, dtype=self._dtype or np.uint32)
         if len(self._parts) > 1:
             self._parts = [np.concatenate(self._parts)]
         return self._parts[0]
@@ -101,7 +109,7 @@ class SliceWriter(Visitor, Clearable):
         self._pos += 1
 
     def visit_many(self, values):
-        v = np.asarray(values, dtype=np.uint32)
+        v = np.asarray(values)
         end = self._pos + v.size
         if end > len(self.data):
             raise IndexError(f"SliceWriter overflow: {end} > {len(self.data)}")
@@ -115,7 +123,7 @@ class SliceWriter(Visitor, Clearable):
         self._pos = 0
 
     def as_array(self):
-        return np.asarray(self.data[: self._pos], dtype=np.uint32)
+        return np.asarray(self.data[: self._pos])
 
 
 class BsrWriter(Clearable):

@@ -107,6 +107,15 @@ def test_argument_errors_before_device_use():
     # batch handle NULL
     assert lib.setops_hip_batch_run(None, None, None, 0, None, None, None, None) == _native.SETOPS_ERR_INVALID
     assert lib.setops_hip_batch_check(None) == _native.SETOPS_ERR_INVALID
+    # the int32 mode is for plain entries: the reference builds BSR from u32 values
+    w, el = ctypes.c_size_t(), ctypes.c_uint64()
+    rc = lib.setops_hip_intersect2_bsr(bsr | _native.SIGNED_I32, p, p, 4, p, p, 4, None, None, 0, ctypes.byref(w),
+                                       ctypes.byref(el))
+    assert rc == _native.SETOPS_ERR_UNSUPPORTED
+    assert b"int32" in lib.setops_hip_last_error()
+    # an unknown id stays unknown with the int32 flag
+    assert lib.setops_hip_intersect2(99 | _native.SIGNED_I32, p, 4, p, 4, None, 0,
+                                     ctypes.byref(n)) == _native.SETOPS_ERR_UNKNOWN_ALGO
 
 
 def test_version_string():

@@ -725,3 +725,71 @@ def test_small_adaptive_many_sets():
         exp = oracle.small_adaptive(sets)
         assert np.array_equal(exp, oracle.svs("branchless_merge", sets))
         assert np.array_equal(intersect.run_kset(sets, "hip_small_adaptive"), exp), trial
+
+
+# ---- ordering against the caller's streams -------------------------------------------------------------
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_branchless_merge"])
+def test_gather_offsets_pending_on_the_callers_stream(name):
+    """A gather batch created (explicit capacity, no host read of the sizes) from device offsets that
+    work still pending on torch's stream writes: creation counts the units only after that work."""
+    import torch
+
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(131))
+    sets, pool, off, ia, ib = _gather_case(rng, 300, 4000, 3000)
+    announce(f"{name} gather offsets pending P={ia.size}")
+    dpool = torch.from_numpy(pool.view(np.int32)).cuda()
+    d_off = torch.from_numpy(off.view(np.int64)).cuda()
+    d_ia = torch.from_numpy(ia.astype(np.int64)).cuda()
+    d_ib = torch.from_numpy(ib.astype(np.int64)).cuda()
+    cap = int(np.minimum(np.diff(off)[ia], np.diff(off)[ib]).sum())
+    for rep in range(3):
+        torch.cuda.synchronize()
+        x = torch.randn(4096, 4096, device="cuda")
+        for _ in range(8):
+            x = x @ x  # keeps the stream busy while the offsets below are still to be written
+            x = x / x.abs().amax()
+        a_beg = torch.empty_like(d_ia)
+        a_end = torch.empty_like(d_ia)
+        b_beg = torch.empty_like(d_ib)
+        b_end = torch.empty_like(d_ib)
+        a_beg.copy_(d_off[d_ia] + (x[0, 0] * 0).long())  # written after the products
+        a_end.copy_(d_off[d_ia + 1])
+        b_beg.copy_(d_off[d_ib])
+        b_end.copy_(d_off[d_ib + 1])
+        gb = GatherBatch(name, dpool, a_beg, a_end, dpool, b_beg, b_end, capacity=cap)
+        gb.run()
+        g_off, g_cnt, g_data = gb.results()
+        gb.close()
+        for p in range(0, ia.size, 7):
+            exp = oracle.branchless_merge(sets[ia[p]], sets[ib[p]])
+            assert g_cnt[p] == exp.size, (rep, p)
+            assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), (rep, p)
+
+
+def test_batch_runs_alternating_streams():
+    """Runs of one handle on two streams in turn share its workspace: each run waits for the last."""
+    import torch
+
+    from simd_set_operations_amd.batch import DeviceBatch
+
+    rng = np.random.Generator(np.random.PCG64(137))
+    As = [sorted_set(rng, int(rng.integers(0, 6000)), 20000) for _ in range(900)]
+    Bs = [sorted_set(rng, int(rng.integers(0, 6000)), 20000) for _ in range(900)]
+    A, a_off = _csr(As)
+    B, b_off = _csr(Bs)
+    counts, off, data = _batch_expect(A, a_off, B, b_off)
+    announce("alternating streams P=900")
+    db = DeviceBatch("hip_branchless_merge", A, a_off, B, b_off)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for rep in range(6):
+        db.run(s1 if rep % 2 == 0 else s2)
+        db.run(s2 if rep % 2 == 0 else s1, count_only=True)
+        db.run(s1 if rep % 2 == 0 else s2)
+        torch.cuda.synchronize()
+        g_off, g_cnt, g_data = db.results()
+        assert np.array_equal(g_cnt, counts), rep
+        assert np.array_equal(g_off, off), rep
+        assert np.array_equal(g_data, data), rep
+    db.close()
