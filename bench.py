@@ -43,6 +43,52 @@ def log(msg):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "merge_tile_kernel_uneven_batch_traffic.txt")
+TRACE_FILE = os.path.join(ROOT, "profiles", "uneven_batch_and_skewed_kernel_trace.txt")
+# Host threads for the CPU baselines: the host cores this process may run on, at most 16 (the
+# host-CPU share a GPU of this pool comes with, whatever the machine's total core count is).
+CPU_SHARE = 16
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(CPU_SHARE, n))
+
+
+def host_info():
+    """What the host offers: logical CPUs, the affinity set, and lscpu's model and topology."""
+    info = {"cpu_count": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes", "CPU max MHz": "max_mhz"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception:
+        pass
+    return info
+
+
+def profiled_avg_us(kernel):
+    """The kernel's average duration (us) in the committed rocprofv3 kernel-trace summary, or None."""
+    try:
+        with open(TRACE_FILE) as f:
+            for line in f:
+                if line.startswith(kernel + " "):
+                    return float(line[len(kernel):].split()[1])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
 
 
 def measured_traffic(kernel):
@@ -65,14 +111,16 @@ def algorithmic_bytes(E, out_total, P):
 
 
 def cpu_baselines(A, a_off, B, b_off, E, budget_s):
-    """Times the reference's QFilter (compiled from its source), the branchless_merge restatement
-    and the croaring comparator's restatement (roaring bitmaps built outside the timed region, as
-    the reference's harness prepares them) over the whole batch, one pair per host thread; each
-    repeats until ~budget_s / 2.  Keys starting with "countonly_" time a count-only pass and are
-    not candidates for the baseline (the GPU value writes every result)."""
+    """Times the reference's QFilter (compiled from its source), the branchless_merge restatement,
+    the reference's widest SIMD block intersection restated with x86 intrinsics (shuffling_avx512,
+    or shuffling_avx2 without AVX-512) and the croaring comparator's restatement (roaring bitmaps
+    built outside the timed region, as the reference's harness prepares them) over the whole
+    batch, one pair per host thread; each repeats until ~budget_s / 2.  Keys starting with
+    "countonly_" time a count-only pass and are not candidates for the baseline (the GPU value
+    writes every result)."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     res = {}
     rbat = oracle.RoaringBatch(A, a_off, B, b_off)
     for key, mat in (("port_croaring_and", True), ("countonly_port_croaring_and_cardinality", False)):
@@ -104,6 +152,16 @@ def cpu_baselines(A, a_off, B, b_off, E, budget_s):
         if el > budget_s / 2 or reps >= 200:
             break
     res["port_branchless_merge"] = {"gelems": E * reps / el / 1e9, "reps": reps, "seconds": el}
+    simd = oracle.fastest_simd_block_algo()
+    if simd:
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            oracle.batch_intersect(simd, A, a_off, B, b_off, nthreads=threads, with_output=True)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 2 or reps >= 200:
+                break
+        res["port_" + simd] = {"gelems": E * reps / el / 1e9, "reps": reps, "seconds": el}
     return res, threads
 
 
@@ -229,8 +287,9 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     symmetrized and deduplicated.  For every edge u < v, |N(u) & N(v)| summed (= 3T), count-only,
     on the CSR in place (gather batches): plain merge, BSR merge (popcount sum), the oriented lists
     (= T, the independent check 3 x T), and a skew split -- galloping for edges whose larger list
-    exceeds `threshold` x the smaller, merge for the rest -- over a small threshold sweep (the
-    threshold is autotuned here, on this graph).  Throughput uses the streamed element count
+    exceeds `threshold` x the smaller, merge for the rest -- over a small threshold sweep that
+    includes the threshold the autotuner persisted in tuning.json ("tuned_split"; "best_split" is
+    the fastest on this graph).  Throughput uses the streamed element count
     sum_{u<v} (d_u + d_v) = sum_v d_v^2."""
     from simd_set_operations_amd import datagen
     from simd_set_operations_amd.graph import EdgeIntersections, csr_to_bsr, split_by_skew
@@ -279,7 +338,14 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     e.close()
     res["merge_bsr"] = {"ms": ms, "sum": s3b, "gelems": E / ms / 1e6}
     best = None
-    for thr, indexed in ((32.0, False), (128.0, False), (512.0, False), (32.0, True), (128.0, True)):
+    from simd_set_operations_amd import tuning
+
+    tuned = tuning.graph_split_threshold()  # measured by the autotuner on a smaller R-MAT graph
+    res["tuned_threshold"] = tuned
+    sweep = [(32.0, False), (128.0, False), (512.0, False), (32.0, True), (128.0, True)]
+    if (tuned, True) not in sweep:
+        sweep.append((tuned, True))
+    for thr, indexed in sweep:
         sk = split_by_skew(deg[u], deg[v], thr)
         a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk, index=indexed)
         b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
@@ -294,6 +360,7 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
         if best is None or ms < res[best]["ms"]:
             best = key
     res["best_split"] = best
+    res["tuned_split"] = f"split_{int(tuned)}_indexed"
     if s3 != 3 * t or s3b != s3:
         log(f"graph: sums disagree (merge {s3}, bsr {s3b}, oriented {t})")
         sys.exit(3)
@@ -479,10 +546,11 @@ def kset_leg(with_cpu):
 def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
     """The served k-set workload: a device-resident corpus of 2^20 sorted lists (bounded-Zipf(1.1)
     lengths in [2^4, 2^18] over [0, 2^26)); queries of k ~ U{2..6} lists by Zipf(1.0) popularity,
-    answered by SvS smallest-first, arriving as Poisson streams.  The continuous-batching engine
-    (every in-flight query's next round in one gather batch per step, admission by arena budget) is
-    compared with the plain path (one query at a time, one launch sequence per round) on the same
-    stream.  Per-query elements follow the reference's convention (sum of the k list sizes)."""
+    answered smallest-first, arriving as Poisson streams.  The continuous-batching engine (each
+    admitted query answered whole in one batch of k-set groups, two steps in flight on two streams,
+    admission against the free HBM less a reserve) is compared with the plain path (one query at a
+    time, one launch sequence per SvS round) on the same stream.  Per-query elements follow the
+    reference's convention (sum of the k list sizes)."""
     from simd_set_operations_amd import serving
 
     t0 = time.perf_counter()
@@ -503,7 +571,7 @@ def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
         eng.serve(wids, wks, serving.poisson_arrivals(2000, rate, seed=400 + i))
         lat, _, st = eng.serve(ids, ks, arr)
         r = {"queries": nq, "engine": serving.latency_summary(lat), "engine_steps": st["steps"],
-             "engine_pairs": st["pairs"], "engine_qps": st["served"] / st["wall_s"],
+             "engine_queries_per_step": st["pairs"] / max(st["steps"], 1), "engine_qps": st["served"] / st["wall_s"],
              "engine_gelems": E / st["wall_s"] / 1e9}
         plat, _, pst = serving.plain_serve(corpus, ids, ks, arr, timeout_s=plain_budget_s)
         r["plain"] = serving.latency_summary(plat)
@@ -660,9 +728,19 @@ def main():
             "sample": f"full {P}-pair batch x {best['reps']} repetitions, one pair per host thread "
                       f"({best_kind})",
         }
+        import oracle
+
+        dram = oracle.stream_read_gbs(threads)
+        cpu["host"] = host_info()
+        cpu["dram_read_gbs"] = dram
         extra["cpu_all"] = res
         extra["speedup_vs_cpu"] = value / best["gelems"]
-        log(f"cpu: {res}")
+        # the ratio beside the two memory rooflines: HBM peak over the host's measured DRAM read
+        extra["cpu_vs_gpu_rooflines"] = {"gpu_hbm_peak_gbs": HBM_PEAK_GBS, "host_dram_read_gbs": dram,
+                                         "bandwidth_ratio": HBM_PEAK_GBS / dram,
+                                         "speedup_vs_cpu": value / best["gelems"],
+                                         "cpu_bytes_per_s_fraction_of_dram": best["gelems"] * 4 / dram}
+        log(f"cpu: {res}, host {cpu['host']}, dram read {dram:.1f} GB/s")
     if not args.no_pair:
         extra["plain_pair"] = pair_leg(torch, not args.no_baselines)
         log(f"plain pair: {extra['plain_pair']}")
@@ -673,11 +751,11 @@ def main():
     if not args.no_serving:
         extra["serving"] = serving_leg(torch, (5e4, 2e5, 8e5), args.serving_seconds, 5.0, args.corpus_mode)
     if not args.no_graph:
-        extra["graph"] = graph_leg(torch, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines,
-                                   args.graph_scale)
+        extra["graph"] = graph_leg(torch, host_threads(), not args.no_baselines, args.graph_scale)
     if not args.no_skewed:
-        extra["skewed"] = skewed_leg(torch, args.steps, max(1, min(16, os.cpu_count() or 1)), not args.no_baselines)
+        extra["skewed"] = skewed_leg(torch, args.steps, host_threads(), not args.no_baselines)
 
+    prof_us = profiled_avg_us(DOMINANT_KERNEL[args.algo])
     rec = {
         "metric": "batched intersect Gelem/s + %HBM roofline; p50/p99 query latency vs plain",
         "value": value,
@@ -712,6 +790,12 @@ def main():
             "kernel": DOMINANT_KERNEL[args.algo],
             "kernel_ms": kern_ms,
             "algorithmic_bytes": Bytes,
+            # the same bytes over the kernel's average in the committed rocprofv3 summary, and over
+            # the whole step (plan, split, tile kernel, scan, placement)
+            "frac_profile": (Bytes / (prof_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if prof_us else None,
+            "profile_avg_us": prof_us,
+            "profile_source": os.path.relpath(TRACE_FILE, ROOT),
+            "step_frac": Bytes / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": cpu,
         "extra": extra,

@@ -65,6 +65,16 @@ const char *setops_hip_version(void);
  * Every entry point that takes an algorithm id accepts it; results are the same int32 values. */
 #define SETOPS_SIGNED_I32 0x10000
 
+/* OR'ed into an algorithm id: the tile shape of the call's kernels (0 = the default, also when
+ * no shape is given).  The plain branchless_merge entry has shapes 0-3 (256 lanes x 20, 16, 24, 12
+ * merge steps per lane), the plain search entries (galloping, binary_search, simd_galloping,
+ * small_adaptive) shapes 0-2 (256 lanes x 2, 1, 4 probes per lane); every other entry shape 0
+ * only (SETOPS_ERR_UNSUPPORTED otherwise).  Results do not depend on the shape; the autotuner
+ * (simd_set_operations_amd/autotune.py) times them.  No reference counterpart. */
+#define SETOPS_SHAPE_SHIFT 20
+#define SETOPS_SHAPE_MASK (0xF << SETOPS_SHAPE_SHIFT)
+#define SETOPS_SHAPE(i) ((i) << SETOPS_SHAPE_SHIFT)
+
 /* Number of registered entries; ids are 0 .. count-1. */
 int setops_hip_algo_count(void);
 /* Id of a registered name (e.g. "hip_branchless_merge"), or SETOPS_ERR_UNKNOWN_ALGO. */
@@ -159,6 +169,21 @@ int setops_hip_batch_create_gather(setops_hip_batch **handle, int algo, const ui
  * afterwards where pairs read them.  Waits for the index build.  No-op for merge-policy entries.
  * No reference counterpart (the reference searches host memory). */
 int setops_hip_batch_index_b(setops_hip_batch *handle);
+
+/* k-set groups in one batch: pair p of a gather batch also needs each of its further sets
+ * B[x_beg[j] .. x_end[j]), j in [x_off[p], x_off[p+1]) -- the group (A_p, B_p, further sets) is one
+ * k-set query, answered in the batch's one launch sequence with no intermediate results: every
+ * element of the pair's intersection is looked up in the further sets in turn, the first that lacks
+ * it ending its lookup -- small_adaptive's walk (setops/src/intersect/adaptive.rs:107-147), and
+ * the result svs_generic's (setops/src/intersect/svs.rs:50-73), the ascending k-way intersection.
+ * d_x_off ([P+1], x_off[0] = 0), d_x_beg and d_x_end are device arrays of pool indexes into B, read
+ * at run time (their contents may change between runs; a rebound batch keeps them).  For the search
+ * entries on plain sets (hip_small_adaptive, hip_galloping, hip_binary_search, hip_simd_galloping);
+ * with an index of B (setops_hip_batch_index_b) a lookup reads the index and one 128-byte line.
+ * All three NULL detaches the further sets.  The reference runs k-set queries one at a time
+ * (benchmark/src/timer/harness.rs:152-172). */
+int setops_hip_batch_set_further(setops_hip_batch *handle, const uint64_t *d_x_off, const uint64_t *d_x_beg,
+                                 const uint64_t *d_x_end);
 
 /* Points a gather batch at new pairs (same pools and algorithm), reusing its device workspace
  * (grow-only: no allocation once it is large enough).  h_na / h_nb are the pairs' sizes on the host

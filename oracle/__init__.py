@@ -4,8 +4,10 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 only as the checker (or the timed CPU baseline) -- never as the thing measured or shipped.
 
 * ``liboracle.so``            -- setops_oracle.c, one C function per reference function (each
-  cites the reference file:line it restates), and roaring.c, a restatement of the croaring
-  comparator the reference's benchmark times beside its own algorithms.
+  cites the reference file:line it restates), roaring.c, a restatement of the croaring
+  comparator the reference's benchmark times beside its own algorithms, and simd_x86.c, the
+  reference's AVX-512 / AVX2 block intersections restated with x86 intrinsics (CPU baseline
+  candidates) and a STREAM-like read of host memory.
 * ``_ref/libqfilter_ref.so``  -- the reference's own setops/ffi/qfilter/qfilter.cpp, compiled
   from its source by oracle/Makefile (absent when the reference checkout was unavailable at
   build time).
@@ -24,6 +26,9 @@ Pinning (what checks each oracle function, see tests/test_oracle.py):
   svs: left fold of numpy.intersect1d == k-way intersection (reduce over all sets).
   roaring_intersect / roaring_svs (roaring.c, the croaring comparator): numpy.intersect1d and
       brute force, on sets that exercise array and bitmap containers across several high keys.
+  shuffling_avx512 / broadcast_avx512 / shuffling_avx2 (simd_x86.c): numpy.intersect1d, brute
+      force and the scalar restatements of the same block walks, on block-aligned and ragged
+      sizes (run where the CPU has the instructions).
 """
 import ctypes
 import os
@@ -45,6 +50,9 @@ QFILTER = 7
 BROADCAST4 = 8
 SIMD_GALLOPING4 = 9
 SIMD_GALLOPING16 = 10
+SHUFFLING_AVX512 = 11  # simd_x86.c: x86 intrinsics (CPU baseline), when oracle_simd_level() allows
+BROADCAST_AVX512 = 12
+SHUFFLING_AVX2 = 13
 
 ALGOS = {
     "naive_merge": NAIVE_MERGE,
@@ -58,6 +66,9 @@ ALGOS = {
     "broadcast_sse": BROADCAST4,
     "galloping_sse": SIMD_GALLOPING4,
     "galloping_avx512": SIMD_GALLOPING16,
+    "shuffling_avx512_simd": SHUFFLING_AVX512,
+    "broadcast_avx512_simd": BROADCAST_AVX512,
+    "shuffling_avx2_simd": SHUFFLING_AVX2,
 }
 
 _c = ctypes
@@ -122,6 +133,15 @@ def lib():
         L.roaring_batch_free.argtypes = [_vp]
         L.roaring_batch_run.restype = _c.c_double
         L.roaring_batch_run.argtypes = [_vp, _c.c_int, _c.c_int, _vp]
+        # simd_x86.c
+        L.oracle_simd_level.restype = _c.c_int
+        L.oracle_simd_level.argtypes = []
+        for n in ("oracle_shuffling_avx512", "oracle_broadcast_avx512", "oracle_shuffling_avx2"):
+            f = getattr(L, n)
+            f.restype = _sz
+            f.argtypes = [_vp, _sz, _vp, _sz, _vp]
+        L.oracle_stream_read.restype = _c.c_double
+        L.oracle_stream_read.argtypes = [_vp, _sz, _c.c_int, _c.c_int, _c.POINTER(_c.c_uint64)]
         L.oracle_batch_intersect_gather.restype = _c.c_int
         L.oracle_batch_intersect_gather.argtypes = [_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
                                                     _vp]
@@ -131,6 +151,29 @@ def lib():
 
 def ref_available():
     return os.path.exists(REF_LIB)
+
+
+def simd_level():
+    """2 when the host has AVX-512F, 1 for AVX2 only, 0 otherwise (CPUID)."""
+    return int(lib().oracle_simd_level())
+
+
+def fastest_simd_block_algo():
+    """The widest x86 block intersection this host runs (shuffling, as the reference's harness
+    selects by target feature), or None."""
+    return {2: "shuffling_avx512_simd", 1: "shuffling_avx2_simd"}.get(simd_level())
+
+
+def stream_read_gbs(nthreads, nbytes=2 << 30, reps=3):
+    """Host DRAM read bandwidth (GB/s): `nthreads` threads each sum a contiguous share of an
+    `nbytes` buffer `reps` times (after one untimed pass)."""
+    buf = np.ones(nbytes // 8, np.uint64)
+    s = _c.c_uint64()
+    lib().oracle_stream_read(_p(buf), buf.size, nthreads, 1, _c.byref(s))
+    el = lib().oracle_stream_read(_p(buf), buf.size, nthreads, reps, _c.byref(s))
+    if s.value != buf.size * reps:
+        raise RuntimeError("stream read checksum")
+    return buf.nbytes * reps / el / 1e9
 
 
 def ref():

@@ -586,7 +586,15 @@ enum {
     ORACLE_BROADCAST4 = 8,
     ORACLE_SIMD_GALLOPING4 = 9,
     ORACLE_SIMD_GALLOPING16 = 10,
+    /* x86 SIMD restatements (simd_x86.c), for the CPU baseline: callers check oracle_simd_level() */
+    ORACLE_SHUFFLING_AVX512 = 11,
+    ORACLE_BROADCAST_AVX512 = 12,
+    ORACLE_SHUFFLING_AVX2 = 13,
 };
+
+size_t oracle_shuffling_avx512(const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out);
+size_t oracle_broadcast_avx512(const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out);
+size_t oracle_shuffling_avx2(const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out);
 
 size_t oracle_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t *b, size_t nb, uint32_t *out)
 {
@@ -602,6 +610,9 @@ size_t oracle_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t 
     case ORACLE_BROADCAST4: return oracle_broadcast(4, a, na, b, nb, out);
     case ORACLE_SIMD_GALLOPING4: return oracle_simd_galloping(4, a, na, b, nb, out);
     case ORACLE_SIMD_GALLOPING16: return oracle_simd_galloping(16, a, na, b, nb, out);
+    case ORACLE_SHUFFLING_AVX512: return oracle_shuffling_avx512(a, na, b, nb, out);
+    case ORACLE_BROADCAST_AVX512: return oracle_broadcast_avx512(a, na, b, nb, out);
+    case ORACLE_SHUFFLING_AVX2: return oracle_shuffling_avx2(a, na, b, nb, out);
     default: return (size_t)-1;
     }
 }

@@ -25,6 +25,11 @@ ALGO_TWOSET = 1
 ALGO_BSR = 2
 ALGO_KSET = 4
 SIGNED_I32 = 0x10000  # OR'ed into an algorithm id: int32 sets, signed order
+SHAPE_SHIFT = 20  # SETOPS_SHAPE(i) = i << SHAPE_SHIFT: the tile shape of a call
+
+
+def shape_flag(i):
+    return int(i) << SHAPE_SHIFT
 
 _ERR_NAMES = {
     SETOPS_ERR_INVALID: "SETOPS_ERR_INVALID",
@@ -80,6 +85,7 @@ HIP_SIGNATURES = {
         [_c.POINTER(_vp), _c.c_int, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp, _c.c_uint64],
     ),
     "setops_hip_batch_index_b": (_c.c_int, [_vp]),
+    "setops_hip_batch_set_further": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "setops_hip_batch_rebind": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp]),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),

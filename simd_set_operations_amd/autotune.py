@@ -1,4 +1,5 @@
-"""Autotuner: batched throughput of every registered 2-set entry over a grid of pair shapes.
+"""Autotuner: batched throughput of every registered 2-set entry and tile shape over a grid of pair
+shapes, and the skew threshold that splits a graph's edges between the search and merge entries.
 
 Grid: larger-set size 2^6 .. 2^14, skew (larger / smaller) 2^0 .. 2^10, selectivity 0.1 / 0.5 /
 0.9, density 0.1 (element space = 10 x larger), the planted-shared-then-disjoint data of the
@@ -17,8 +18,11 @@ import time
 
 import numpy as np
 
-ALGOS = ("hip_branchless_merge", "hip_shuffling", "hip_qfilter", "hip_galloping", "hip_binary_search",
-         "hip_simd_galloping")
+# entries with distinct kernels, and the tile shapes of those that have several ("name@shape":
+# merge steps per lane 20/16/24/12, probes per lane 2/1/4; include/setops_hip.h SETOPS_SHAPE)
+ALGOS = ("hip_branchless_merge", "hip_branchless_merge@1", "hip_branchless_merge@2", "hip_branchless_merge@3",
+         "hip_shuffling", "hip_qfilter", "hip_galloping", "hip_galloping@1", "hip_galloping@2", "hip_binary_search",
+         "hip_binary_search@1", "hip_binary_search@2", "hip_simd_galloping")
 GRID = {
     "log2_larger": [6, 8, 10, 12, 14],
     "log2_skew": [0, 3, 6, 10],
@@ -87,6 +91,7 @@ def run(out_path, runs=5, grid=None, algos=ALGOS, log=print):
                               "smaller": smaller, "pairs": pairs, "elements": E, "gelems": g, "best": best})
                 log(f"[autotune] L=2^{l} skew=2^{s} sel={sel} pairs={pairs}: best {best} "
                     + " ".join(f"{a[4:]}={v:.0f}" for a, v in g.items()))
+    graph = graph_split(torch, log=log)
     table = {
         "device": torch.cuda.get_device_name(0),
         "created_unix": int(time.time()),
@@ -97,10 +102,62 @@ def run(out_path, runs=5, grid=None, algos=ALGOS, log=print):
         "grid": grid,
         "algorithms": list(algos),
         "cells": cells,
+        "graph_skew_split": graph,
     }
     with open(out_path, "w") as f:
         json.dump(table, f, indent=1)
     return table
+
+
+GRAPH_SCALE = 20
+GRAPH_THRESHOLDS = (8.0, 16.0, 32.0, 64.0, 128.0, 512.0)
+
+
+def graph_split(torch, scale=GRAPH_SCALE, thresholds=GRAPH_THRESHOLDS, reps=3, log=print):
+    """The skew threshold for graph edge intersections: edges whose larger neighbour list is more
+    than T times the smaller go to hip_galloping over the indexed CSR, the rest to
+    hip_branchless_merge (count-only, as the triangle count runs them).  Times every T on an
+    R-MAT graph (0.57, 0.19, 0.19, 0.05) at 2^scale vertices, edge factor 16, checks that every
+    split gives the plain merge's sum, and returns the fastest T with the timings."""
+    from . import datagen
+    from .graph import EdgeIntersections, split_by_skew
+
+    rp, col = datagen.rmat_graph(scale=scale)
+    u, v = datagen.graph_edges(rp, col)
+    deg = np.diff(rp).astype(np.int64)
+    dcol = torch.from_numpy(col.view(np.int32)).cuda()
+    stream = torch.cuda.current_stream()
+
+    def timed(batches):
+        for b in batches:
+            b.run(stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            for b in batches:
+                b.run(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, sum(b.totals()[0] for b in batches)
+
+    plain = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v)
+    ms_plain, total = timed([plain])
+    plain.close()
+    res = {"scale": scale, "edges": int(u.size), "plain_merge_ms": ms_plain, "ms": {}}
+    for thr in thresholds:
+        sk = split_by_skew(deg[u], deg[v], thr)
+        a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk, index=True)
+        b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
+        ms, t = timed([a, b])
+        a.close()
+        b.close()
+        if t != total:
+            raise RuntimeError(f"graph split at {thr}: sum {t} differs from the plain merge's {total}")
+        res["ms"][str(thr)] = ms
+        log(f"[autotune] graph 2^{scale} skew split at {thr:g}: {ms:.3f} ms (plain merge {ms_plain:.3f} ms)")
+    res["threshold"] = float(min(res["ms"], key=lambda k: res["ms"][k]))
+    return res
 
 
 def main():

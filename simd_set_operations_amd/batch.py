@@ -46,8 +46,9 @@ def to_device_u64(arr, torch):
 
 
 def resolve_algo(algo, a_sizes, b_sizes, bsr):
-    """``hip_auto``: the entry the tuning table prices cheapest for these pair sizes (its BSR twin
-    for BSR sets); any other name is returned as it is."""
+    """``hip_auto``: the entry and tile shape (``name@shape``) the tuning table prices cheapest for
+    these pair sizes (its BSR twin, in the default shape, for BSR sets); any other name is
+    returned as it is."""
     if algo != "hip_auto":
         return algo
     from . import tuning
@@ -56,6 +57,7 @@ def resolve_algo(algo, a_sizes, b_sizes, bsr):
     if bsr:
         from .intersect import registered
 
+        best = best.partition("@")[0]
         twin = best + "_bsr"
         return twin if twin in registered() else "hip_galloping_bsr" if best == "hip_binary_search" else \
             "hip_branchless_merge_bsr"
@@ -260,3 +262,52 @@ class GatherBatch(DeviceBatch):
         )
         self.handle = h
         self._outputs(capacity if outputs else 0)
+
+    # -- k-set groups --------------------------------------------------------------------------
+    def set_further(self, x_off, x_beg=None, x_end=None):
+        """k-set groups: pair p's results must also be in every further set B[x_beg[j]:x_end[j]],
+        j in [x_off[p], x_off[p+1]) (setops_hip_batch_set_further; search entries, plain sets).
+        ``x_off=None`` detaches the further sets."""
+        lib = _native.hip()
+        if x_off is None:
+            _native.check(lib.setops_hip_batch_set_further(self.handle, None, None, None))
+            self._further = None
+            return self
+        torch = self.torch
+
+        def dev(a):
+            if _is_tensor(a):
+                return a if a.numel() else torch.zeros(1, dtype=torch.int64, device="cuda")
+            a = np.ascontiguousarray(a, dtype=np.uint64)
+            return to_device_u64(a if a.size else np.zeros(1, np.uint64), torch)
+
+        self._further = (dev(x_off), dev(x_beg), dev(x_end))
+        _native.check(lib.setops_hip_batch_set_further(self.handle, *(t.data_ptr() for t in self._further)))
+        return self
+
+
+class KsetBatch(GatherBatch):
+    """k-set groups over one pool, all answered in one batch: group g is the sets
+    pool[beg[i]:end[i]] for i in [group_off[g], group_off[g+1]) (k >= 2 each), in the caller's order
+    (smallest first, as svs_generic takes them, setops/src/intersect/svs.rs:50-73).  The first two
+    sets of a group are its pair; the others are its further sets (set_further).  Results: one
+    ascending k-way intersection per group, in group order (``results()``)."""
+
+    def __init__(self, algo, pool, beg, end, group_off, index=False, capacity=None, signed=None):
+        beg = np.ascontiguousarray(beg, np.uint64)
+        end = np.ascontiguousarray(end, np.uint64)
+        group_off = np.ascontiguousarray(group_off, np.int64)
+        G = group_off.size - 1
+        k = np.diff(group_off)
+        if G < 1 or np.any(k < 2):
+            raise ValueError("every group needs at least two sets")
+        first = group_off[:-1]
+        further = np.ones(beg.size, bool)
+        further[first] = False
+        further[first + 1] = False
+        x_off = (group_off - 2 * np.arange(G + 1)).astype(np.uint64)
+        super().__init__(algo, pool, beg[first], end[first], pool, beg[first + 1], end[first + 1], capacity=capacity,
+                         signed=signed)
+        if index:
+            self.index_b()
+        self.set_further(x_off, beg[further], end[further])

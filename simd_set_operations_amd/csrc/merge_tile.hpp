@@ -75,6 +75,33 @@ __device__ __forceinline__ uint32_t mp_split(const uint32_t *key, uint32_t la, u
     return lo;
 }
 
+// One merge step on the current A value a and B value b (branchless_merge, merge.rs:39-60): the
+// step takes A when a <= b (ties go to A) and matches when a == b.  Both masks gain the step's bit
+// at the bottom, doubling as they go (step k of K ends at bit K-1-k), and ia counts the A elements
+// taken.  Each compare result is the carry-in of an add (v_addc_co_u32 with VCC): one VALU per mask
+// bit.  Then the next A element's byte offset oa = a4 + 4 ia and bo = s4 - oa, from which the
+// caller forms the next B element's offset (s4 + 4(k+1) - oa): seven VALU per step.  s_nop 1 gives
+// the two wait states that a VALU read of VCC as a carry needs after a VALU write of it.  Pure
+// register arithmetic (no memory access).
+__device__ __forceinline__ void merge_step(uint32_t a, uint32_t b, uint32_t a4, uint32_t s4, uint32_t &tmask,
+                                           uint32_t &smask, uint32_t &ia, uint32_t &oa, uint32_t &bo)
+{
+    uint64_t c0, c1, c2;  // carry-outs (unused)
+    asm("v_cmp_le_u32 vcc, %[a], %[b]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[t], %[c0], %[t], %[t], vcc\n\t"
+        "v_addc_co_u32_e64 %[i], %[c1], %[i], 0, vcc\n\t"
+        "v_cmp_eq_u32 vcc, %[a], %[b]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[s], %[c2], %[s], %[s], vcc\n\t"
+        "v_lshl_add_u32 %[oa], %[i], 2, %[a4]\n\t"
+        "v_sub_u32 %[bo], %[s4], %[oa]"
+        : [t] "+v"(tmask), [i] "+v"(ia), [s] "+v"(smask), [oa] "=&v"(oa), [bo] "=v"(bo), [c0] "=&s"(c0),
+          [c1] "=&s"(c1), [c2] "=&s"(c2)
+        : [a] "v"(a), [b] "v"(b), [a4] "v"(a4), [s4] "v"(s4)
+        : "vcc");
+}
+
 // Bytes of x that are zero -> 0x80 in that byte, else 0 (exact, no carries across bytes).
 __device__ __forceinline__ uint32_t zero_bytes(uint32_t x)
 {
@@ -139,7 +166,8 @@ __device__ __forceinline__ uint32_t block_intersect(const uint32_t *key, const u
 template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP>
 __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR, GATHER, LP != LP_MERGE> &sm,
                                                    const TilePlan &plan, const TileOut &out, uint32_t t,
-                                                   uint32_t nseg, uint32_t ulen, uint32_t p_first, uint32_t in_sflip)
+                                                   uint32_t nseg, uint32_t ulen, uint32_t p_first, uint32_t in_sflip,
+                                                   uint32_t nstaged)
 {
     constexpr bool BLOCKP = LP != LP_MERGE;
     using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
@@ -150,8 +178,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         // ---- 3. per-lane merge ---------------------------------------------------------------------
         // smask: bit k set when step k emitted; tmask: bit k set when step k consumed an A element.
         // The emitted values are A elements, re-read from LDS at write time: the A offset of the
-        // emit at step k is popcount(tmask & ((1 << k) - 1)) from the lane's start (pa0), or, after
-        // the lane moved to the next pair at step kt, from that pair's window start (pa1).
+        // emit at step k is popcount(tmask & ((1 << k) - 1)) from where the lane's merge starts
+        // (pa0: its own pair, or the next pair's window when it starts there at step kt).
         const uint32_t ub = tid * K;
         uint32_t smask = 0, tmask = 0, pa0 = 0, pa1 = 0;
         uint32_t nelems = 0;
@@ -287,27 +315,25 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             }
             const uint32_t kv = kmax - shift;
             pa0 = pa;
-            pa1 = pa;
             pbl = pb;
             shift_o = shift;
             uint32_t a = sm.key[pa], b = sm.key[pb];
-            // oa: byte offset of A's current element; B's is s4 + 4*steps - oa.  The empty asm
-            // keeps oa the one position variable (otherwise the compiler rebuilds both offsets
-            // from separate counters every step).
+            // A's current element is at byte 4(pa + ia) of key, B's at 4(pb + k - ia): their sum
+            // is s4 + 4k.  The masks come out of the loop with step k at bit K-1-k.
             const char *kb = reinterpret_cast<const char *>(sm.key);
-            const uint32_t s4 = 4u * (pa + pb);
-            uint32_t oa = 4u * pa;
+            const uint32_t s4 = 4u * (pa + pb), a4 = 4u * pa;
+            uint32_t ia = 0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const bool takeA = a <= b;
-                smask |= (a == b) ? (1u << k) : 0u;
-                tmask |= takeA ? (1u << k) : 0u;
-                oa += takeA ? 4u : 0u;
-                asm volatile("" : "+v"(oa));
+                uint32_t oa, bo;
+                merge_step(a, b, a4, s4, tmask, smask, ia, oa, bo);
                 a = *reinterpret_cast<const uint32_t *>(kb + oa);
-                // B's byte offset 4(pb + k + 1 - taken A) >= 4 pb: formed in 32-bit arithmetic first
-                b = *reinterpret_cast<const uint32_t *>(kb + ((s4 + 4u * (uint32_t)(k + 1)) - oa));
+                // B's byte offset 4(pb + k + 1 - taken A) >= 4 pb (bo + 4(k+1), 32-bit arithmetic)
+                b = *reinterpret_cast<const uint32_t *>(kb + (bo + 4u * (uint32_t)(k + 1)));
             }
+            // step k to bit k
+            tmask = __builtin_bitreverse32(tmask) >> (32 - K);
+            smask = __builtin_bitreverse32(smask) >> (32 - K);
             smask &= (kv >= 32u) ? ~0u : ((1u << kv) - 1u);
             const uint32_t rA = ea - pa;
             const uint32_t rB = (ebp > eb) ? ~0u : eb - pb;
@@ -365,6 +391,15 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         }
         __syncthreads();
 
+        // int32 sets: the keys go back to their given bits before they are written out (the merges
+        // that compared them are done: the scan above waited for every wave)
+        if constexpr (!COUNT_ONLY && !BSR) {
+            if (in_sflip) {
+                for (uint32_t i = tid; i < nstaged; i += NT) sm.key[i] ^= in_sflip;
+                __syncthreads();
+            }
+        }
+
         // ---- 5. staged outputs (in order inside the tile's slot) ----------------------------------
         // Sorted unique windows give at most (K + 1) / 2 results per lane, which the slot holds; a
         // tile planned from sizes that disagree with the device offsets may not, and then writes
@@ -378,12 +413,6 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         if constexpr (!COUNT_ONLY) {
             uint32_t *dk = plan.stage + (uint64_t)t * plan.stage_words;
             uint32_t *ds = dk + plan.stage_words / 2;
-            const uint32_t flip = in_sflip;  // LDS holds keys XOR flip; results are the keys as given
-            const uint32_t kt_mask = (kt >= 32u) ? ~0u : ((1u << kt) - 1u);
-            auto a_index = [&](uint32_t k) -> uint32_t {
-                const uint32_t before = tmask & ((1u << k) - 1u);
-                return (k < kt) ? pa0 + __popc(before) : pa1 + __popc(before & ~kt_mask);
-            };
             uint32_t pos = pfx;
             if constexpr (BLOCKP) {
                 uint32_t sm_ = smask;
@@ -392,7 +421,7 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                     sm_ &= sm_ - 1;
                     const uint32_t ai = (x < nfirst) ? pa0 + x : pa1 + (x - nfirst);
                     const uint32_t v = sm.key[ai];
-                    dk[pos] = v ^ flip;
+                    dk[pos] = v;
                     if constexpr (BSR) {
                         // the matching B base lies in the same sub-window: find it again
                         const uint32_t jb = (x < nfirst) ? pb0 : pb1, nbw = (x < nfirst) ? nb0 : nb1;
@@ -416,11 +445,17 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
                     pos++;
                 }
             } else {
+                // the A element taken at step k is the lane's A element number popc(tmask below k),
+                // counted from pa0 (a lane that moved to the next pair started there and has no
+                // steps below kt); results are written at 32-bit byte offsets from the slot
+                char *dkb = reinterpret_cast<char *>(dk);
+                uint32_t off = 4u * pos;
                 uint32_t sm_ = smask;
                 while (sm_) {
                     const uint32_t k = __builtin_ctz(sm_);
                     sm_ &= sm_ - 1;
-                    dk[pos++] = sm.key[a_index(k)] ^ flip;
+                    *reinterpret_cast<uint32_t *>(dkb + off) = sm.key[pa0 + __popc(tmask & ((1u << k) - 1u))];
+                    off += 4u;
                 }
             }
         }
@@ -487,6 +522,15 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         const uint32_t t = tile0 + blockIdx.x;
         if (t >= plan.ntiles) return;
         // ---- 1. descriptor and segment windows -------------------------------------------------
+        // CSR: the tile's A and B runs load with its pair range (one round trip before the
+        // staging is issued, not two)
+        uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
+        if constexpr (!GATHER) {
+            ga0 = plan.tile_ga[t];
+            ga1 = plan.tile_ga[t + 1];
+            gb0 = plan.tile_gb[t];
+            gb1 = plan.tile_gb[t + 1];
+        }
         const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
@@ -501,14 +545,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         if constexpr (!GATHER) {
             // CSR: the tile's A run and B run come straight from the split (issued before the
             // segment table's loads)
-            uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
-            if (!bad) {
-                ga0 = plan.tile_ga[t];
-                ga1 = plan.tile_ga[t + 1];
-                gb0 = plan.tile_gb[t];
-                gb1 = plan.tile_gb[t + 1];
-                bad = ga1 < ga0 || gb1 < gb0 || (ga1 - ga0) + (gb1 - gb0) > (uint64_t)U;
-            }
+            if (!bad) bad = ga1 < ga0 || gb1 < gb0 || (ga1 - ga0) + (gb1 - gb0) > (uint64_t)U;
             const uint32_t n_a = bad ? 0u : (uint32_t)(ga1 - ga0);
             const uint32_t n_b = bad ? 0u : (uint32_t)(gb1 - gb0) + ((gb1 < in.b_pool) ? 1u : 0u);
             const uint32_t shift_a = (uint32_t)(ga0 & 3), shift_b = (uint32_t)(gb0 & 3);
@@ -559,6 +596,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                 sm.seg_lb[tid] = 4u * pb_ + (uint32_t)(gb & 3);
             }
             if (tid == 0) sm.win_c[2 * nseg] = totA + totB;
+            staged_words = 4u * (totA + totB);
             __syncthreads();
             constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
             if (!bad) {
@@ -594,7 +632,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         }
 
         merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first,
-                                                                       in.sflip);
+                                                                       in.sflip, staged_words);
     }
 }
 

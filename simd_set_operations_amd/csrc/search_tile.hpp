@@ -551,6 +551,45 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 hmask |= found ? (1u << k) : 0u;
             }
         }
+        if constexpr (!BSR) {
+            // k-set groups: each hit is looked up in its pair's further sets in B, in order; the
+            // first that lacks it drops it (small_adaptive's walk, adaptive.rs:107-147, one element
+            // at a time: each lookup searches the whole further set).  A pass of its own after the
+            // lookups, so its state is not live beside theirs.
+            if (in.x_off && hmask) {
+                uint32_t s = s0;
+                for (int k = 0; k < KS; k++) {
+                    if (!((hmask >> k) & 1u)) continue;
+                    const uint32_t u = ub + (uint32_t)k;
+                    while (u >= sm.seg_u[s + 1]) s++;
+                    const uint32_t x = sm.pkey[sm.seg_la[s] + (u - sm.seg_u[s])];
+                    const uint64_t p = (uint64_t)p_first + s;
+                    const uint64_t j1 = in.x_off[p + 1];
+                    bool found = true;
+                    for (uint64_t j = in.x_off[p]; found && j < j1; j++) {
+                        // a range is read only inside the pool (and its index)
+                        const uint64_t g1 = min(in.x_end[j], in.b_pool), g0 = min(in.x_beg[j], g1);
+                        if (in.b_idx) {
+                            found = index_probe(in.B, in.b_pool, in.b_idx, g0, g1, x, in.sflip) != kNoPos;
+                        } else if (g1 > g0) {
+                            const uint32_t n = (uint32_t)(g1 - g0);
+                            if constexpr (SMODE == SM_BLOCK) {
+                                (void)lower_bound_block_pool(in.B, in.b_pool, g0, 0u, n, x, in.sflip, found);
+                            } else {
+                                const uint32_t *__restrict__ X = in.B + g0;
+                                const uint32_t f = in.sflip;
+                                auto xarr = [&](uint32_t i) { return X[i] ^ f; };
+                                const uint32_t xp = lower_bound_from<false>(xarr, 0u, n, x);
+                                found = xp < n && xarr(xp) == x;
+                            }
+                        } else {
+                            found = false;
+                        }
+                    }
+                    if (!found) hmask &= ~(1u << k);
+                }
+            }
+        }
         const uint32_t nwords = __popc(hmask);
         if constexpr (!BSR) nelems = nwords;
 

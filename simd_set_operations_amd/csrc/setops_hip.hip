@@ -49,20 +49,23 @@ int fail(int code, const char *fmt, ...)
 // ---------------------------------------------------------------------------------------------
 // kernel configuration
 // ---------------------------------------------------------------------------------------------
-// merge policy: kNTM lanes x kK merge-path steps per tile
+// merge policy: kNTM lanes x K merge-path steps per tile; search policy: kNT lanes x KS probes.
+// The tile shape is selected per call (SETOPS_SHAPE(i) in the algorithm id): shape 0 is the
+// default, the others exist for the plain merge and search entries, for the autotuner to pick.
 constexpr int kNT = 256;  // workgroup size of the search tile
 constexpr int kNTM = 256;
-constexpr int kK = 20;
-constexpr int kU = kNTM * kK;
+constexpr int kNumShapes = 4;
+constexpr int kMergeK[kNumShapes] = {20, 16, 24, 12};  // merge steps per lane
+constexpr int kSearchKS[kNumShapes] = {2, 1, 4, 0};    // probes per lane (0: no such shape)
+constexpr int kK = kMergeK[0];
+constexpr int kKS = kSearchKS[0];
 constexpr int kMaxSeg = 130;
-// every pair owns at least c0 units so a tile touches at most kMaxSeg pairs
-constexpr uint32_t kC0 = (kU + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
-// search policy: 256 lanes x 2 probes per tile, larger windows staged up to kCapB words
-constexpr int kKS = 2;
-constexpr int kUS = kNT * kKS;
-constexpr int kCapB = 3072;
-constexpr uint32_t kC0S = (kUS + (kMaxSeg - 2) - 1) / (kMaxSeg - 2);
-static_assert(kC0 >= (uint32_t)kK && kC0S >= (uint32_t)kKS, "a lane crosses at most one pair boundary");
+// every pair owns at least c0 = ceil(U / (kMaxSeg - 2)) units, so a tile touches at most kMaxSeg
+// pairs; c0 = 2K (2KS) >= K (KS): a lane crosses at most one pair boundary
+constexpr uint32_t pad_for(uint32_t U) { return (U + (kMaxSeg - 2) - 1) / (kMaxSeg - 2); }
+static_assert(pad_for(kNTM * 12) >= 12 && pad_for(kNTM * 20) >= 20 && pad_for(kNT * 1) >= 1,
+              "a lane crosses at most one pair boundary");
+constexpr int kCapB = 3072;  // search policy: larger windows staged up to this many words
 
 constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this many groups run as one kernel
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
@@ -124,7 +127,7 @@ struct Context {
     // grow-only device buffers for the host-pointer entry points
     void *host_buf = nullptr;
     size_t host_buf_bytes = 0;
-    setops_hip_batch *single[kNumAlgos] = {};  // cached 1-pair handles per algorithm
+    setops_hip_batch *single[kNumAlgos][kNumShapes] = {};  // cached 1-pair handles per algorithm and shape
 };
 
 Context g_ctx;
@@ -161,22 +164,30 @@ int ensure_init(int device)
         if (rc_ != SETOPS_OK) return rc_;      \
     } while (0)
 
-// Registered entry of an algorithm id; SETOPS_SIGNED_I32 may be OR'ed into the id.
+// Registered entry of an algorithm id; SETOPS_SIGNED_I32 and SETOPS_SHAPE(i) may be OR'ed into it.
+constexpr int kAlgoFlags = SETOPS_SIGNED_I32 | SETOPS_SHAPE_MASK;
 const AlgoEntry *algo_entry(int algo)
 {
-    algo &= ~SETOPS_SIGNED_I32;
+    algo &= ~kAlgoFlags;
     if (algo < 0 || algo >= kNumAlgos) return nullptr;
     return &kAlgos[algo];
 }
-int algo_base(int algo) { return algo & ~SETOPS_SIGNED_I32; }
+int algo_base(int algo) { return algo & ~kAlgoFlags; }
+int shape_of(int algo) { return (algo & SETOPS_SHAPE_MASK) >> SETOPS_SHAPE_SHIFT; }
 // PairBatch::sflip of an algorithm id: int32 sets compare with bit 31 flipped.
 uint32_t sflip_of(int algo) { return (algo & SETOPS_SIGNED_I32) ? 0x80000000u : 0u; }
 // int32 mode exists for plain entries only: the reference's BSR form is built from u32 values
 // (benchmark/src/timer/harness.rs:121-122 casts the i32 sets to u32 before BsrVec::from_sorted).
-int check_signed(int algo, const AlgoEntry *e)
+int shape_count(Kind k, bool bsr);
+// The flags OR'ed into an algorithm id must fit its entry: the int32 mode for plain entries, a
+// tile shape the entry offers.
+int check_flags(int algo, const AlgoEntry *e)
 {
     if ((algo & SETOPS_SIGNED_I32) && e->bsr)
         return fail(SETOPS_ERR_UNSUPPORTED, "%s: BSR entries take u32 bases (no int32 mode)", e->name);
+    if (shape_of(algo) >= shape_count(e->kind, e->bsr))
+        return fail(SETOPS_ERR_UNSUPPORTED, "%s has no tile shape %d (it has %d)", e->name, shape_of(algo),
+                    shape_count(e->kind, e->bsr));
     return SETOPS_OK;
 }
 
@@ -189,6 +200,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // ---------------------------------------------------------------------------------------------
 struct setops_hip_batch {
     int algo = 0;
+    int shape = 0;  // tile shape (index into kMergeK / kSearchKS)
     Kind kind = KIND_MERGE;
     bool bsr = false;
     bool gather = false;
@@ -231,13 +243,26 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles);
 // Merge-path kinds (merge, shuffling, qfilter) share the merge tile; the search kinds the search tile.
 bool merge_like(Kind k) { return k == KIND_MERGE || k == KIND_SHUFFLE || k == KIND_QFILTER; }
 uint32_t policy_of(Kind k) { return merge_like(k) ? POLICY_MERGE : POLICY_SEARCH; }
-uint32_t units_per_tile(Kind k) { return merge_like(k) ? (uint32_t)kU : (uint32_t)kUS; }
-uint32_t pad_units(Kind k) { return merge_like(k) ? kC0 : kC0S; }
+// Tile shapes a kind offers: the plain merge entry and the plain search entries have several,
+// every other entry (block policies, BSR) has the default only.
+int shape_count(Kind k, bool bsr)
+{
+    if (bsr) return 1;
+    if (k == KIND_MERGE) return kNumShapes;
+    if (!merge_like(k)) return 3;
+    return 1;
+}
+uint32_t units_per_tile(Kind k, int shape)
+{
+    return merge_like(k) ? (uint32_t)(kNTM * kMergeK[shape]) : (uint32_t)(kNT * kSearchKS[shape]);
+}
+uint32_t pad_units(Kind k, int shape) { return pad_for(units_per_tile(k, shape)); }
 // Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
 // can produce (merge: (U+1)/2 + 1; search: U).
-uint32_t stage_words(Kind k, bool bsr)
+uint32_t stage_words(Kind k, bool bsr, int shape)
 {
-    const uint32_t half = (merge_like(k) ? (uint32_t)kU / 2 : (uint32_t)kUS) + 64;
+    const uint32_t U = units_per_tile(k, shape);
+    const uint32_t half = (merge_like(k) ? U / 2 : U) + 64;
     return bsr ? 2 * half : half;
 }
 // The plan's unit prefix has a closed form only for merge-policy CSR batches (and single pairs).
@@ -246,7 +271,7 @@ bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gath
 // Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields.
 int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
 {
-    const uint64_t U = units_per_tile(h->kind);
+    const uint64_t U = units_per_tile(h->kind, h->shape);
     const uint64_t ntiles = std::max<uint64_t>(1, (utotal + U - 1) / U);
     if (ntiles >= 0x7FFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu tiles", (unsigned long long)ntiles);
     if (P >= 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
@@ -264,7 +289,7 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
     }
     h->plan.utotal = utotal;
     h->plan.ntiles = (uint32_t)ntiles;
-    h->plan.c0 = pad_units(h->kind);
+    h->plan.c0 = pad_units(h->kind, h->shape);
     h->plan.U = (uint32_t)U;
     h->plan.policy = policy_of(h->kind);
     return SETOPS_OK;
@@ -280,7 +305,7 @@ size_t ws_layout(setops_hip_batch *h, uint64_t P, uint64_t ntiles, char *b)
         off = align_up(off + bytes, 256);
         return o;
     };
-    const uint32_t sw = stage_words(h->kind, h->bsr);
+    const uint32_t sw = stage_words(h->kind, h->bsr, h->shape);
     const size_t o_uoff = take((P + 1) * 8);
     const size_t o_tf = take(ntiles * 4);
     const size_t o_tl = take(ntiles * 4);
@@ -407,22 +432,52 @@ int device_unit_total(setops_hip_batch *h, uint64_t *utotal)
 
 // Tile kernels run one workgroup per tile; batches of more than kMaxGrid tiles take several
 // launches (tile offset tile0).
-template <bool BSR, bool CO, bool GA, int LP>
+template <int K, bool BSR, bool CO, bool GA, int LP>
 void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((merge_tile_kernel<kNTM, kK, kMaxSeg, BSR, CO, GA, LP>),
+        hipLaunchKernelGGL((merge_tile_kernel<kNTM, K, kMaxSeg, BSR, CO, GA, LP>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNTM), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
 }
 
-template <bool BSR, bool CO, int SMODE>
+template <int KS, bool BSR, bool CO, int SMODE>
 void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((search_tile_kernel<kNT, kKS, kMaxSeg, kCapB, BSR, CO, SMODE>),
+        hipLaunchKernelGGL((search_tile_kernel<kNT, KS, kMaxSeg, kCapB, BSR, CO, SMODE>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
+}
+
+// The plain merge entry in the handle's tile shape (other merge entries have the default only).
+template <bool BSR, bool CO, bool GA>
+void launch_merge_shape(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    if constexpr (BSR) {
+        launch_merge_t<kMergeK[0], BSR, CO, GA, LP_MERGE>(h, out, stream);
+    } else {
+        switch (h->shape) {
+        case 1: launch_merge_t<kMergeK[1], BSR, CO, GA, LP_MERGE>(h, out, stream); break;
+        case 2: launch_merge_t<kMergeK[2], BSR, CO, GA, LP_MERGE>(h, out, stream); break;
+        case 3: launch_merge_t<kMergeK[3], BSR, CO, GA, LP_MERGE>(h, out, stream); break;
+        default: launch_merge_t<kMergeK[0], BSR, CO, GA, LP_MERGE>(h, out, stream); break;
+        }
+    }
+}
+
+template <bool BSR, bool CO, int SMODE>
+void launch_search_shape(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
+{
+    if constexpr (BSR) {
+        launch_search_t<kSearchKS[0], BSR, CO, SMODE>(h, out, stream);
+    } else {
+        switch (h->shape) {
+        case 1: launch_search_t<kSearchKS[1], BSR, CO, SMODE>(h, out, stream); break;
+        case 2: launch_search_t<kSearchKS[2], BSR, CO, SMODE>(h, out, stream); break;
+        default: launch_search_t<kSearchKS[0], BSR, CO, SMODE>(h, out, stream); break;
+        }
+    }
 }
 
 template <bool BSR, bool CO>
@@ -430,21 +485,21 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     switch (h->kind) {
     case KIND_MERGE:
-        if (h->gather) launch_merge_t<BSR, CO, true, LP_MERGE>(h, out, stream);
-        else launch_merge_t<BSR, CO, false, LP_MERGE>(h, out, stream);
+        if (h->gather) launch_merge_shape<BSR, CO, true>(h, out, stream);
+        else launch_merge_shape<BSR, CO, false>(h, out, stream);
         break;
     case KIND_SHUFFLE:
-        if (h->gather) launch_merge_t<BSR, CO, true, LP_SHUFFLE>(h, out, stream);
-        else launch_merge_t<BSR, CO, false, LP_SHUFFLE>(h, out, stream);
+        if (h->gather) launch_merge_t<kK, BSR, CO, true, LP_SHUFFLE>(h, out, stream);
+        else launch_merge_t<kK, BSR, CO, false, LP_SHUFFLE>(h, out, stream);
         break;
     case KIND_QFILTER:
-        if (h->gather) launch_merge_t<BSR, CO, true, LP_QFILTER>(h, out, stream);
-        else launch_merge_t<BSR, CO, false, LP_QFILTER>(h, out, stream);
+        if (h->gather) launch_merge_t<kK, BSR, CO, true, LP_QFILTER>(h, out, stream);
+        else launch_merge_t<kK, BSR, CO, false, LP_QFILTER>(h, out, stream);
         break;
     case KIND_GALLOP:
-    case KIND_ADAPTIVE: launch_search_t<BSR, CO, SM_GALLOP>(h, out, stream); break;
-    case KIND_BINARY: launch_search_t<BSR, CO, SM_BINARY>(h, out, stream); break;
-    case KIND_BLOCK_GALLOP: launch_search_t<BSR, CO, SM_BLOCK>(h, out, stream); break;
+    case KIND_ADAPTIVE: launch_search_shape<BSR, CO, SM_GALLOP>(h, out, stream); break;
+    case KIND_BINARY: launch_search_shape<BSR, CO, SM_BINARY>(h, out, stream); break;
+    case KIND_BLOCK_GALLOP: launch_search_shape<BSR, CO, SM_BLOCK>(h, out, stream); break;
     }
 }
 
@@ -611,7 +666,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     *handle = nullptr;
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
-    if (int rc = check_signed(algo, e)) return rc;
+    if (int rc = check_flags(algo, e)) return rc;
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
     const bool gather = d_a_end != nullptr;
     if (!d_a_beg || !d_b_beg || (gather && !d_b_end)) return fail(SETOPS_ERR_INVALID, "offset arrays must not be NULL");
@@ -625,6 +680,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     }
     auto *h = new setops_hip_batch();
     h->algo = algo_base(algo);
+    h->shape = shape_of(algo);
     h->kind = e->kind;
     h->bsr = e->bsr;
     h->gather = gather;
@@ -646,7 +702,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->plan.direct = 0;
     int rc;
     if (!gather && merge_like(e->kind)) {
-        rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)kC0 * npairs);
+        rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)pad_units(h->kind, h->shape) * npairs);
     } else {
         // size the scan arrays first, then count units on the device
         rc = batch_reserve(h, npairs, 1);
@@ -713,10 +769,11 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
                   hipStream_t s, const uint32_t *const *dX = nullptr, const uint64_t *dXlen = nullptr, uint32_t nx = 0)
 {
     const AlgoEntry *e = algo_entry(algo);
-    setops_hip_batch *&h = g_ctx.single[algo_base(algo)];
+    setops_hip_batch *&h = g_ctx.single[algo_base(algo)][shape_of(algo)];
     if (!h) {
         h = new setops_hip_batch();
         h->algo = algo_base(algo);
+        h->shape = shape_of(algo);
         h->kind = e->kind;
         h->bsr = e->bsr;
     }
@@ -724,7 +781,7 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
     const uint64_t units =
-        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind);
+        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, h->shape);
     h->plan.single = 1;
     int rc = batch_reserve(h, 1, units);
     if (rc != SETOPS_OK) return rc;
@@ -926,7 +983,7 @@ int setops_hip_intersect2(int algo, const uint32_t *a, size_t na, const uint32_t
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_TWOSET)) return fail(SETOPS_ERR_UNSUPPORTED, "%s is not a plain 2-set entry", e->name);
-    if (int rc = check_signed(algo, e)) return rc;
+    if (int rc = check_flags(algo, e)) return rc;
     if ((na && !a) || (nb && !b)) return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if (!out_len) return fail(SETOPS_ERR_INVALID, "out_len is NULL");
     ENSURE_INIT();
@@ -942,7 +999,7 @@ int check_dev_pair(int algo, const uint32_t *d_a, const uint32_t *d_a_states, si
 {
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
-    if (int rc = check_signed(algo, e)) return rc;
+    if (int rc = check_flags(algo, e)) return rc;
     if ((na && !d_a) || (nb && !d_b)) return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if (!aligned16(d_a) || !aligned16(d_b)) return fail(SETOPS_ERR_INVALID, "inputs must be 16-byte aligned");
     if (!d_totals) return fail(SETOPS_ERR_INVALID, "d_totals is NULL");
@@ -978,7 +1035,7 @@ int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t 
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_BSR)) return fail(SETOPS_ERR_UNSUPPORTED, "%s is not a BSR entry", e->name);
-    if (int rc = check_signed(algo, e)) return rc;
+    if (int rc = check_flags(algo, e)) return rc;
     if ((na && (!bases_a || !states_a)) || (nb && (!bases_b || !states_b)))
         return fail(SETOPS_ERR_INVALID, "input pointer is NULL");
     if ((bases_out == nullptr) != (states_out == nullptr))
@@ -996,7 +1053,7 @@ int setops_hip_svs(int algo, const uint32_t *const *sets, const size_t *lens, si
     const AlgoEntry *e = algo_entry(algo);
     if (!e) return fail(SETOPS_ERR_UNKNOWN_ALGO, "unknown algorithm id %d", algo);
     if (!(e->flags & SETOPS_ALGO_KSET)) return fail(SETOPS_ERR_UNSUPPORTED, "%s cannot drive svs_generic", e->name);
-    if (int rc = check_signed(algo, e)) return rc;
+    if (int rc = check_flags(algo, e)) return rc;
     if (k < 2) return fail(SETOPS_ERR_INVALID, "svs needs at least 2 sets (svs.rs:61)");
     if (!sets || !lens || !out_len) return fail(SETOPS_ERR_INVALID, "NULL argument");
     for (size_t s = 0; s < k; s++)
@@ -1106,6 +1163,29 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
     return SETOPS_OK;
 }
 
+int setops_hip_batch_set_further(setops_hip_batch *handle, const uint64_t *d_x_off, const uint64_t *d_x_beg,
+                                 const uint64_t *d_x_end)
+{
+    if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
+    if (!d_x_off && !d_x_beg && !d_x_end) {
+        handle->in.x_off = nullptr;
+        handle->in.x_beg = nullptr;
+        handle->in.x_end = nullptr;
+        return SETOPS_OK;
+    }
+    if (!d_x_off || !d_x_beg || !d_x_end) return fail(SETOPS_ERR_INVALID, "further-set arrays must all be set or all NULL");
+    if (!handle->gather) return fail(SETOPS_ERR_UNSUPPORTED, "further sets need a gather batch");
+    const AlgoEntry *e = algo_entry(handle->algo);
+    if (merge_like(handle->kind) || handle->bsr)
+        return fail(SETOPS_ERR_UNSUPPORTED,
+                    "%s: further sets are looked up by the search entries on plain sets (hip_small_adaptive, "
+                    "hip_galloping, hip_binary_search, hip_simd_galloping)", e->name);
+    handle->in.x_off = d_x_off;
+    handle->in.x_beg = d_x_beg;
+    handle->in.x_end = d_x_end;
+    return SETOPS_OK;
+}
+
 int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, const uint64_t *d_a_end,
                             const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs, const uint64_t *h_na,
                             const uint64_t *h_nb)
@@ -1115,7 +1195,7 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
     if (!d_a_beg || !d_a_end || !d_b_beg || !d_b_end || !h_na || !h_nb)
         return fail(SETOPS_ERR_INVALID, "NULL argument");
-    const uint32_t c0 = pad_units(handle->kind);
+    const uint32_t c0 = pad_units(handle->kind, handle->shape);
     uint64_t utotal = 0;
     for (uint64_t p = 0; p < npairs; p++) {
         if (h_na[p] >= 0xFFFFFFFFull || h_nb[p] >= 0xFFFFFFFFull)

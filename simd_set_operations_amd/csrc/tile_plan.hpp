@@ -52,6 +52,12 @@ struct PairBatch {
     const uint32_t *const *xsets;
     const uint64_t *xlens;
     uint32_t nx;
+    // optional, any number of pairs, search policy, plain sets: pair p's further sets are
+    // B[x_beg[j] .. x_end[j]) for j in [x_off[p], x_off[p+1]) (k-set groups: a probe found in the
+    // pair's larger set is a result only if every further set holds it too)
+    const uint64_t *x_off;
+    const uint64_t *x_beg;
+    const uint64_t *x_end;
 };
 
 constexpr int kMaxExtraSets = 8;  // further sets per launch (k-set queries run in groups of 2 + 8)

@@ -67,7 +67,10 @@ def init(device=0):
 
 
 def algo_id(name):
-    return _native.check(_native.hip().setops_hip_algo_lookup(name.encode()))
+    """Id of a registered entry; ``"name@i"`` selects tile shape i (SETOPS_SHAPE)."""
+    base, _, shape = name.partition("@")
+    aid = _native.check(_native.hip().setops_hip_algo_lookup(base.encode()))
+    return aid | (_native.shape_flag(int(shape)) if shape else 0)
 
 
 def registered():
@@ -170,7 +173,7 @@ def lookup(name):
 
 def __getattr__(attr):
     # module-level access to registered entries: intersect.hip_branchless_merge(a, b, v)
-    if attr.startswith("hip_"):
+    if attr.startswith("hip_") and "@" not in attr:
         try:
             return lookup(attr)
         except _native.SetopsError as e:

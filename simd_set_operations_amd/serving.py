@@ -1,21 +1,24 @@
-"""Served k-set queries over a device-resident corpus: continuous batching of SvS rounds.
+"""Served k-set queries over a device-resident corpus: continuous batching of k-set groups.
 
-A corpus of sorted u32 lists stays in one device pool.  A query names k lists; it is answered by
-SvS smallest-first (svs_generic, setops/src/intersect/svs.rs:50-73): the running result meets the
-next list, k-1 rounds.  The engine runs the rounds of *all* in-flight queries as one gather batch
-per step:
+A corpus of sorted u32 lists stays in one device pool.  A query names k lists, smallest first; its
+answer is their intersection, what SvS gives (svs_generic, setops/src/intersect/svs.rs:50-73).  The
+engine answers every admitted query completely in one step: a step is one batch of k-set groups
+(setops_hip_batch_set_further) -- the query's first two lists are a pair of a gather batch and its
+other lists are that pair's further sets, in which every element of the pair's intersection is
+looked up in turn (small_adaptive's walk, setops/src/intersect/adaptive.rs:107-147).
 
-* new arrivals are admitted while their worst-case intermediate results fit the free-HBM budget
-  (an arena of device words, double-buffered) and a pair slot is free;
-* every active query contributes one pair to the step -- (list 0, list 1) in its first round, then
-  (its last result in the arena, next list) -- so queries at different rounds overlap on the device;
-* after the step the host reads the per-pair counts and offsets, finishes the queries whose last
-  round this was (or whose result became empty: every later round is empty too), and moves the
-  others on.  A finished query's result is copied to the host in the same step.
+* Two steps are in flight, on two streams, each with its own batch handle and output buffer: while
+  the device runs one step, the host admits the queries that arrived meanwhile and launches the
+  next (the device may run both at once); then it waits for the older step, finishes its queries
+  (results and offsets come back by copies enqueued behind the step) and frees their memory.
+* Admission is in arrival order against a device-memory budget -- the free HBM at start-up less a
+  stated reserve -- with each query holding its worst-case cost while in flight, and against the
+  pair slots of a step.
 
-Latency is measured from a query's (Poisson) arrival time to the end of the step that finished it.
-The plain baseline answers the same query stream one query at a time with one launch sequence per
-round (setops_hip_intersect2_dev), each round synchronised to size the next.
+Latency is measured from a query's (Poisson) arrival time to the host's view of the end of the step
+that answered it.  The plain baseline answers the same query stream one query at a time with one
+launch sequence per SvS round (setops_hip_intersect2_dev), each round synchronised to size the
+next.
 """
 import ctypes
 import time
@@ -53,11 +56,10 @@ def zipf_lengths(n, lmin, lmax, s, seed, mode="iid"):
 
 
 class Corpus:
-    """Sorted unique lists generated in device memory, followed by a result arena (two halves) in
-    the same pool, so one gather batch can read corpus lists and earlier results alike."""
+    """Sorted unique lists generated in device memory, in one pool (each list 16-byte aligned)."""
 
     def __init__(self, nlists=CORPUS["nlists"], lmin=CORPUS["lmin"], lmax=CORPUS["lmax"], zipf_s=CORPUS["zipf_s"],
-                 universe=CORPUS["universe"], seed=CORPUS["seed"], mode="iid", arena_words=1 << 26):
+                 universe=CORPUS["universe"], seed=CORPUS["seed"], mode="iid"):
         import torch
 
         self.torch = torch
@@ -72,11 +74,7 @@ class Corpus:
         self.total = int(self.beg[-1] + padded[-1])
         self.nlists = nlists
         self.universe = universe
-        self.half = int(arena_words) // 2
-        self.arena = [self.total + (-self.total % 64), 0]
-        self.arena[1] = self.arena[0] + self.half + (-self.half % 64)
-        words = self.arena[1] + self.half + 64
-        self.pool = torch.empty(words, dtype=torch.int32, device="cuda")
+        self.pool = torch.empty(self.total + 64, dtype=torch.int32, device="cuda")
         d_beg = torch.from_numpy(self.beg.view(np.int64)).cuda()
         d_len = torch.from_numpy(self.lens.view(np.int64)).cuda()
         rc = _native.datagen_hip().setops_gen_sorted_lists_dev(seed, universe, d_beg.data_ptr(), d_len.data_ptr(),
@@ -125,103 +123,188 @@ def poisson_arrivals(nq, rate, seed):
     return np.cumsum(gaps)
 
 
+class Admission:
+    """Admission of arriving queries against a device-memory budget and a pair-slot limit (host
+    logic only).  A query reserves ``cost`` words while it is in flight; queries are admitted in
+    arrival order, and the first that does not fit stops the admission (no overtaking)."""
+
+    def __init__(self, budget_words, max_pairs):
+        self.budget = int(budget_words)
+        self.max_pairs = int(max_pairs)
+        self.reserved = 0
+        self.active = 0
+
+    def admit(self, pending, cost, limit=None):
+        """Pops from the deque `pending` the queries that fit now (at most `limit`)."""
+        add = []
+        room = self.max_pairs if limit is None else min(self.max_pairs, limit)
+        while pending and self.active + len(add) < self.max_pairs and len(add) < room:
+            q = pending[0]
+            c = int(cost[q])
+            if self.reserved + c > self.budget:
+                break
+            pending.popleft()
+            self.reserved += c
+            add.append(q)
+        self.active += len(add)
+        return add
+
+    def release(self, words, n):
+        self.reserved -= int(words)
+        self.active -= int(n)
+
+
+def query_cost_words(corpus, ids):
+    """Worst-case device words a query holds in flight: its result (at most its smallest list,
+    min(|L0|, |L1|) with the lists smallest-first), the result's staging and the plan of its
+    probe tiles (about two more words per probe), plus a fixed 256 words."""
+    need = corpus.lens[np.maximum(ids[:, 0], 0)].astype(np.int64)
+    return need, 3 * need + 256
+
+
+class _Slot:
+    """One step in flight: a gather batch handle over the corpus pool with its own stream, output
+    buffer and host staging (grow-only)."""
+
+    def __init__(self, eng):
+        torch = eng.torch
+        self.torch = torch
+        lib = eng.lib
+        self.stream = torch.cuda.Stream()
+        h = _c.c_void_p()
+        one = torch.tensor([0, 0, 0, 0], dtype=torch.int64, device="cuda")
+        pool = eng.c.pool
+        _native.check(lib.setops_hip_batch_create_gather(
+            _c.byref(h), algo_id(eng.algo), pool.data_ptr(), None, pool.numel(), one.data_ptr(), one.data_ptr() + 8,
+            pool.data_ptr(), None, pool.numel(), one.data_ptr() + 16, one.data_ptr() + 24, 1))
+        self.handle = h
+        self._one = one
+        self.d_idx = self.h_idx = None
+        self.out = self.h_res = None
+        self.d_off = torch.zeros(eng.max_pairs + 1, dtype=torch.int64, device="cuda")
+        self.d_cnt = torch.zeros(eng.max_pairs, dtype=torch.int32, device="cuda")
+        self.d_tot = torch.zeros(2, dtype=torch.int64, device="cuda")
+        self.h_off = torch.zeros(eng.max_pairs + 1, dtype=torch.int64).pin_memory()
+        self.done = torch.cuda.Event()
+        self.queries = None
+
+    def _grow(self, words_idx, words_out):
+        torch = self.torch
+        if self.d_idx is None or self.d_idx.numel() < words_idx:
+            n = max(words_idx, 1 << 16)
+            self.d_idx = torch.empty(n, dtype=torch.int64, device="cuda")
+            self.h_idx = torch.empty(n, dtype=torch.int64).pin_memory()
+        if self.out is None or self.out.numel() < words_out:
+            n = max(words_out, 1 << 20)
+            self.out = torch.empty(n, dtype=torch.int32, device="cuda")
+            self.h_res = torch.empty(n, dtype=torch.int32).pin_memory()
+
+    def close(self, lib):
+        if self.handle:
+            lib.setops_hip_batch_destroy(self.handle)
+            self.handle = None
+
+
 class Engine:
-    """Continuous-batching SvS engine over a Corpus (one gather batch handle, rebound every step).
+    """Continuous batching of k-set queries over a resident Corpus.
 
-    ``index``: index the corpus pool once (every 32nd element), so search-policy entries probe long
-    lists in global memory with about one 128-byte line per lookup."""
+    A step is one batch of k-set groups (setops_hip_batch_set_further): each admitted query is the
+    pair (list 0, list 1) with its lists 2 .. k-1 as further sets, so it is answered completely in
+    that step (no intermediate results, no host round trip per list).  Two steps are in flight on
+    two streams: while the device runs one, the host admits the queries that arrived meanwhile and
+    launches the next, which can run beside it; then it waits for the older step and finishes its
+    queries.  Admission is by the device memory free at start-up (``budget_bytes``, default: the
+    free HBM less ``reserve_bytes``) and by pair slots per step (``max_pairs``).
 
-    def __init__(self, corpus, algo="hip_galloping", max_pairs=16384, index=True):
+    ``index``: index the corpus pool once (every 32nd element), so the search entries probe long
+    lists and further sets with about one 128-byte line per lookup."""
+
+    def __init__(self, corpus, algo="hip_galloping", max_pairs=16384, index=True, reserve_bytes=16 << 30,
+                 budget_bytes=None):
         torch = corpus.torch
         self.c = corpus
         self.torch = torch
         self.algo = algo
-        self.max_pairs = max_pairs
-        lib = _native.hip()
-        self.lib = lib
-        self.d_idx = torch.zeros(4 * max_pairs, dtype=torch.int64, device="cuda")
-        self.h_idx = torch.zeros(4 * max_pairs, dtype=torch.int64).pin_memory()
-        self.d_out_off = torch.zeros(max_pairs + 1, dtype=torch.int64, device="cuda")
-        self.d_counts = torch.zeros(max_pairs, dtype=torch.int32, device="cuda")
-        self.d_tot = torch.zeros(2, dtype=torch.int64, device="cuda")
-        self.h_off = torch.zeros(max_pairs + 1, dtype=torch.int64).pin_memory()
-        self.stream = torch.cuda.Stream()
-        # a gather handle over the whole pool, first bound to one dummy pair
-        h = _c.c_void_p()
-        one = torch.tensor([0, 0, 0, 0], dtype=torch.int64, device="cuda")
-        pool = corpus.pool.data_ptr()
-        _native.check(lib.setops_hip_batch_create_gather(
-            _c.byref(h), algo_id(algo), pool, None, corpus.pool.numel(), one.data_ptr(), one.data_ptr() + 8, pool, None,
-            corpus.pool.numel(), one.data_ptr() + 16, one.data_ptr() + 24, 1))
-        self.handle = h
-        self._keep = one
+        self.max_pairs = int(max_pairs)
+        self.lib = _native.hip()
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info()
+            budget_bytes = free - int(reserve_bytes)
+        if budget_bytes <= 0:
+            raise ValueError(f"no device memory left for serving (free HBM minus a {reserve_bytes} byte reserve)")
+        self.budget_bytes = int(budget_bytes)
+        self.slots = [_Slot(self), _Slot(self)]
         if index:
-            # the B side of every round is a corpus list, never the arena, so an index built
-            # once over the pool stays valid where it is read
-            _native.check(lib.setops_hip_batch_index_b(h))
+            # every pair and every further set reads corpus lists, which never change
+            for sl in self.slots:
+                _native.check(self.lib.setops_hip_batch_index_b(sl.handle))
 
     def close(self):
-        if self.handle:
-            self.lib.setops_hip_batch_destroy(self.handle)
-            self.handle = None
+        for sl in self.slots:
+            sl.close(self.lib)
 
-    def _step(self, a_beg, a_len, b_beg, b_len, half):
-        """One batched round for len(a_beg) pairs; returns (out_off[P+1], results on host)."""
+    def _launch(self, sl, qs, ids, ks, need):
+        """Enqueue one step for queries `qs` on slot `sl` (nothing waits here)."""
         torch = self.torch
-        P = a_beg.size
-        h = self.h_idx.numpy().view(np.uint64)
-        h[0:P] = a_beg
-        h[P:2 * P] = a_beg + a_len
-        h[2 * P:3 * P] = b_beg
-        h[3 * P:4 * P] = b_beg + b_len
-        s = self.stream
-        with torch.cuda.stream(s):
-            self.d_idx[:4 * P].copy_(self.h_idx[:4 * P], non_blocking=True)
-            base = self.d_idx.data_ptr()
-            na = np.ascontiguousarray(a_len, np.uint64)
-            nb = np.ascontiguousarray(b_len, np.uint64)
-            _native.check(self.lib.setops_hip_batch_rebind(
-                self.handle, base, base + 8 * P, base + 16 * P, base + 24 * P, P,
-                na.ctypes.data_as(_c.c_void_p), nb.ctypes.data_as(_c.c_void_p)))
-            out_ptr = self.c.pool.data_ptr() + 4 * self.c.arena[half]
-            _native.check(self.lib.setops_hip_batch_run(
-                self.handle, out_ptr, None, self.c.half, self.d_out_off.data_ptr(), self.d_counts.data_ptr(),
-                self.d_tot.data_ptr(), s.cuda_stream))
-            self.h_off[:P + 1].copy_(self.d_out_off[:P + 1], non_blocking=True)
-        s.synchronize()
-        off = self.h_off[:P + 1].numpy().view(np.uint64).copy()
-        total = int(off[-1])
-        a0 = self.c.arena[half]
-        res = self.c.pool[a0:a0 + total].cpu().numpy().view(np.uint32) if total else np.zeros(0, np.uint32)
-        return off, res
+        c = self.c
+        P = len(qs)
+        qs = np.asarray(qs, np.int64)
+        k = ks[qs]
+        l0, l1 = ids[qs, 0], ids[qs, 1]
+        nx = k - 2
+        X = int(nx.sum())
+        xq = np.repeat(np.arange(P), nx)
+        xj = np.arange(X) - np.repeat(np.cumsum(nx) - nx, nx) + 2
+        xl = ids[qs[xq], xj] if X else np.zeros(0, np.int64)
+        out_words = int(need[qs].sum()) + 64
+        words = 5 * P + 1 + 2 * X
+        sl._grow(words, out_words)
+        h = sl.h_idx.numpy().view(np.uint64)
+        h[0:P] = c.beg[l0]
+        h[P:2 * P] = c.beg[l0] + c.lens[l0]
+        h[2 * P:3 * P] = c.beg[l1]
+        h[3 * P:4 * P] = c.beg[l1] + c.lens[l1]
+        h[4 * P] = 0
+        np.cumsum(nx, out=h[4 * P + 1:5 * P + 1])
+        h[5 * P + 1:5 * P + 1 + X] = c.beg[xl]
+        h[5 * P + 1 + X:5 * P + 1 + 2 * X] = c.beg[xl] + c.lens[xl]
+        na = np.ascontiguousarray(c.lens[l0], np.uint64)
+        nb = np.ascontiguousarray(c.lens[l1], np.uint64)
+        lib = self.lib
+        with torch.cuda.stream(sl.stream):
+            sl.d_idx[:words].copy_(sl.h_idx[:words], non_blocking=True)
+            base = sl.d_idx.data_ptr()
+            _native.check(lib.setops_hip_batch_rebind(
+                sl.handle, base, base + 8 * P, base + 16 * P, base + 24 * P, P, na.ctypes.data_as(_c.c_void_p),
+                nb.ctypes.data_as(_c.c_void_p)))
+            xb = base + 8 * (5 * P + 1)
+            _native.check(lib.setops_hip_batch_set_further(sl.handle, base + 32 * P, xb, xb + 8 * X))
+            _native.check(lib.setops_hip_batch_run(
+                sl.handle, sl.out.data_ptr(), None, sl.out.numel(), sl.d_off.data_ptr(), sl.d_cnt.data_ptr(),
+                sl.d_tot.data_ptr(), sl.stream.cuda_stream))
+            sl.h_off[:P + 1].copy_(sl.d_off[:P + 1], non_blocking=True)
+            sl.h_res[:out_words].copy_(sl.out[:out_words], non_blocking=True)
+            sl.done.record(sl.stream)
+        sl.queries = qs
 
     def serve(self, ids, ks, arrivals, keep_results=False, timeout_s=60.0):
         """Serve the query stream; returns (latency seconds per query, results list or None, stats)."""
         import gc
 
-        c = self.c
         nq = ids.shape[0]
-        lens = c.lens
+        need, cost = query_cost_words(self.c, ids)
+        adm = Admission(self.budget_bytes // 4, self.max_pairs)
+        if np.any(cost > adm.budget):
+            raise ValueError("a query needs more device memory than the serving budget")
         gc_was = gc.isenabled()
         gc.collect()
         gc.disable()  # no collector pauses inside the timed loop
-        need = lens[np.maximum(ids[:, 0], 0)].astype(np.int64)  # worst-case intermediate words
-        if np.any(need > c.half):
-            raise ValueError("a query's smallest list exceeds the arena half")
         done_t = np.full(nq, np.nan)
         results = [None] * nq if keep_results else None
         pending = deque()
         nxt = 0
-        # active state (numpy, one row per active query)
-        act_q = np.zeros(0, np.int64)
-        act_r = np.zeros(0, np.int64)  # rounds done
-        act_ab = np.zeros(0, np.uint64)
-        act_al = np.zeros(0, np.uint64)
-        reserved = 0
-        half = 0
-        steps = 0
-        pairs = 0
-        ndone = 0
+        steps = pairs = ndone = 0
+        cur = 0
         t0 = time.perf_counter()
         while ndone < nq:
             now = time.perf_counter() - t0
@@ -230,49 +313,38 @@ class Engine:
             while nxt < nq and arrivals[nxt] <= now:
                 pending.append(nxt)
                 nxt += 1
-            # admit in arrival order within the budget and the pair slots
-            add = []
-            while pending and act_q.size + len(add) < self.max_pairs and reserved + need[pending[0]] <= c.half:
-                q = pending.popleft()
-                reserved += int(need[q])
-                add.append(q)
-            if add:
-                aq = np.array(add, np.int64)
-                l0 = ids[aq, 0]
-                act_q = np.concatenate([act_q, aq])
-                act_r = np.concatenate([act_r, np.zeros(aq.size, np.int64)])
-                act_ab = np.concatenate([act_ab, c.beg[l0]])
-                act_al = np.concatenate([act_al, lens[l0]])
-            if act_q.size == 0:
-                if nxt < nq:
-                    wait = arrivals[nxt] - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 1e-3))
-                continue
-            lb = ids[act_q, act_r + 1]
-            off, res = self._step(act_ab, act_al, c.beg[lb], lens[lb], half)
-            steps += 1
-            pairs += act_q.size
-            t_end = time.perf_counter() - t0
-            cnt = np.diff(off)
-            act_r += 1
-            fin = (act_r >= ks[act_q] - 1) | (cnt == 0)
-            if fin.any():
-                fq = act_q[fin]
-                done_t[fq] = t_end
-                ndone += fq.size
-                reserved -= int(need[fq].sum())
+            sl, other = self.slots[cur], self.slots[cur ^ 1]
+            if pending:  # this slot is free: its last step was finished below
+                add = adm.admit(pending, cost)
+                if add:
+                    self._launch(sl, add, ids, ks, need)
+                    steps += 1
+                    pairs += len(add)
+            if other.queries is not None:
+                other.done.synchronize()
+                t_end = time.perf_counter() - t0
+                qs = other.queries
+                off = other.h_off[:qs.size + 1].numpy().view(np.uint64)
+                done_t[qs] = t_end
+                ndone += qs.size
+                adm.release(int(cost[qs].sum()), qs.size)
                 if keep_results:
-                    for j in np.flatnonzero(fin):
-                        results[act_q[j]] = res[int(off[j]):int(off[j + 1])].copy()
-            keep = ~fin
-            act_q, act_r = act_q[keep], act_r[keep]
-            act_ab = (np.uint64(c.arena[half]) + off[:-1])[keep]
-            act_al = cnt[keep]
-            half ^= 1
+                    res = other.h_res.numpy().view(np.uint32)
+                    for j, q in enumerate(qs.tolist()):
+                        results[q] = res[int(off[j]):int(off[j + 1])].copy()
+                other.queries = None
+            elif sl.queries is None and not pending and nxt < nq:
+                wait = arrivals[nxt] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 1e-3))
+            cur ^= 1
+        for sl in self.slots:  # a timed-out stream leaves no step behind
+            if sl.queries is not None:
+                sl.done.synchronize()
+                sl.queries = None
         lat = done_t - arrivals
         stats = {"steps": steps, "pairs": pairs, "served": int(np.isfinite(done_t).sum()),
-                 "wall_s": time.perf_counter() - t0}
+                 "wall_s": time.perf_counter() - t0, "budget_bytes": self.budget_bytes}
         if gc_was:
             gc.enable()
         return lat, results, stats

@@ -5,7 +5,9 @@ throughput of each registered 2-set entry measured on one MI355X.  A batch is pr
 by a cost model over that table: every pair is binned to its nearest (size, skew) cell, and the
 predicted time of an entry is the sum over cells of the cell's elements divided by the entry's
 throughput there (averaged over the selectivities, which a caller does not know in advance).
-``choose`` returns the cheapest entry; ``hip_auto`` batches resolve to it at creation.
+``choose`` returns the cheapest entry, in the tile shape the table prices cheapest
+(``name@shape``); ``hip_auto`` batches resolve to it at creation.  The table also holds the
+graph skew-split threshold (``graph_split_threshold``).
 """
 import json
 import os
@@ -80,3 +82,14 @@ def choose(a_sizes, b_sizes, table=None, algos=None):
     """The entry with the smallest predicted batch time."""
     pred = predict(a_sizes, b_sizes, table, algos)
     return min(pred, key=pred.get)
+
+
+def graph_split_threshold(table=None, default=32.0):
+    """The skew threshold the autotuner measured for graph edge intersections (larger list over
+    smaller beyond which an edge goes to the search entry), or `default` without a measurement."""
+    try:
+        table = table or load()
+    except FileNotFoundError:
+        return default
+    g = table.get("graph_skew_split")
+    return float(g["threshold"]) if g and "threshold" in g else default

@@ -113,6 +113,13 @@ def test_argument_errors_before_device_use():
                                        ctypes.byref(el))
     assert rc == _native.SETOPS_ERR_UNSUPPORTED
     assert b"int32" in lib.setops_hip_last_error()
+    # tile shapes: the plain merge entry has four, the block-compare entries one
+    shuf = lib.setops_hip_algo_lookup(b"hip_shuffling")
+    assert lib.setops_hip_intersect2(shuf | _native.shape_flag(1), p, 4, p, 4, None, 0,
+                                     ctypes.byref(n)) == _native.SETOPS_ERR_UNSUPPORTED
+    assert b"tile shape" in lib.setops_hip_last_error()
+    assert lib.setops_hip_intersect2(algo | _native.shape_flag(4), p, 4, p, 4, None, 0,
+                                     ctypes.byref(n)) == _native.SETOPS_ERR_UNSUPPORTED
     # an unknown id stays unknown with the int32 flag
     assert lib.setops_hip_intersect2(99 | _native.SIGNED_I32, p, 4, p, 4, None, 0,
                                      ctypes.byref(n)) == _native.SETOPS_ERR_UNKNOWN_ALGO

@@ -505,7 +505,7 @@ def test_served_queries_alone_and_in_the_mix():
     per-query path's result, and its result when served alone."""
     from simd_set_operations_amd import serving
 
-    corpus = serving.Corpus(nlists=4096, lmin=16, lmax=1 << 12, universe=1 << 18, seed=77, arena_words=1 << 22)
+    corpus = serving.Corpus(nlists=4096, lmin=16, lmax=1 << 12, universe=1 << 18, seed=77)
     for i in range(0, corpus.nlists, 511):
         lst = corpus.list(i)
         assert lst.size == corpus.lens[i]
@@ -516,7 +516,7 @@ def test_served_queries_alone_and_in_the_mix():
     eng = serving.Engine(corpus, "hip_galloping", max_pairs=256)
     lat, res, stats = eng.serve(ids, ks, arrivals, keep_results=True)
     assert stats["served"] == ids.shape[0]
-    assert stats["steps"] > 1 and stats["pairs"] >= ids.shape[0]
+    assert stats["steps"] > 1 and stats["pairs"] == ids.shape[0]  # one step answers a query
     lists = {}
 
     def L(i):
@@ -793,3 +793,42 @@ def test_batch_runs_alternating_streams():
         assert np.array_equal(g_off, off), rep
         assert np.array_equal(g_data, data), rep
     db.close()
+
+
+# ---- tile shapes (SETOPS_SHAPE): the autotuner's alternatives give the same results ------------------
+@pytest.mark.parametrize("name", ["hip_branchless_merge@1", "hip_branchless_merge@2", "hip_branchless_merge@3",
+                                  "hip_galloping@1", "hip_galloping@2", "hip_binary_search@1",
+                                  "hip_binary_search@2", "hip_simd_galloping@1", "hip_small_adaptive@2"])
+def test_tile_shapes_exact(name):
+    from simd_set_operations_amd.batch import DeviceBatch, GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(149))
+    As, Bs = [], []
+    for p in range(900):
+        n_l = int(2 ** rng.uniform(0, 14))
+        n_s = max(0, int(n_l / 2 ** rng.uniform(0, 10)))
+        u = max(10 * n_l, 16)
+        As.append(sorted_set(rng, n_s, u))
+        Bs.append(sorted_set(rng, n_l, u))
+    A, a_off = _csr(As)
+    B, b_off = _csr(Bs)
+    counts, off, data = _batch_expect(A, a_off, B, b_off)
+    announce(f"shape {name} CSR P=900")
+    db = DeviceBatch(name, A, a_off, B, b_off)
+    for rep in range(2):
+        db.run()
+        g_off, g_cnt, g_data = db.results()
+        assert np.array_equal(g_cnt, counts) and np.array_equal(g_off, off) and np.array_equal(g_data, data), rep
+    db.close()
+    sets, pool, soff, ia, ib = _gather_case(rng, 300, 3000, 4000)
+    gb = GatherBatch(name, pool, soff[ia], soff[ia + 1], pool, soff[ib], soff[ib + 1])
+    gb.run()
+    g_off, g_cnt, g_data = gb.results()
+    gb.close()
+    for p in range(0, ia.size, 3):
+        exp = oracle.branchless_merge(sets[ia[p]], sets[ib[p]])
+        assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), p
+    for case in range(6):  # single pairs through the host-pointer call, one to many tiles
+        a = sorted_set(rng, int(rng.integers(0, 1 << (6 + 2 * case))), 1 << 20)
+        b = sorted_set(rng, int(rng.integers(0, 1 << (6 + 2 * case))), 1 << 20)
+        assert np.array_equal(intersect.run_2set(a, b, name), oracle.branchless_merge(a, b)), case

@@ -76,8 +76,9 @@ def test_int32_typed_entry_point(hip):
                                        vp(b.ctypes.data), b.size, vp(out.ctypes.data), out.size, ctypes.byref(n))
     assert rc == 0, hip.setops_hip_last_error()
     assert np.array_equal(out[:n.value], expect(a, b))
-    # the same sets read as u32 give the u32 order's intersection: the same elements
-    got_u = intersect.run_2set(a.view(np.uint32), b.view(np.uint32), "hip_branchless_merge")
+    # the same bits sorted as u32 sets give the same elements, in u32 order
+    au, bu = np.sort(a.view(np.uint32)), np.sort(b.view(np.uint32))
+    got_u = intersect.run_2set(au, bu, "hip_branchless_merge")
     assert np.array_equal(np.sort(got_u.view(np.int32)), expect(a, b))
 
 

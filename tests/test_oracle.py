@@ -343,3 +343,48 @@ def test_roaring_svs_and_batch():
         assert rb.run(nthreads, materialize=mat) >= 0
         assert np.array_equal(rb.counts[:150], want)
     rb.close()
+
+
+# -- x86 SIMD restatements (simd_x86.c), the CPU baseline's block intersections ----------------------
+SIMD_X86 = [("shuffling_avx512_simd", 2, "shuffling_avx512"), ("broadcast_avx512_simd", 2, "shuffling_avx512"),
+            ("shuffling_avx2_simd", 1, "shuffling_avx2")]
+
+
+@pytest.mark.parametrize("algo,level,scalar", SIMD_X86)
+def test_x86_simd_restatements(algo, level, scalar):
+    """shuffling_avx512 / broadcast_avx512 / shuffling_avx2 with intrinsics: the reference's known
+    answers, the golden vectors, numpy on block-aligned and ragged sizes, brute force on a
+    48-element universe (so whole blocks are compared, not only the merge tail), and the scalar
+    restatement of the same block walk."""
+    if oracle.simd_level() < level:
+        pytest.skip("this host lacks the instructions")
+    g = golden()
+    for i in range(6):
+        assert np.array_equal(oracle.intersect2(algo, g[f"kat{i}_a"], g[f"kat{i}_b"]), g[f"kat{i}_out"]), i
+    for i, a, b in golden_pairs(g):
+        assert np.array_equal(oracle.intersect2(algo, a, b), g[f"pair{i}_out"]), i
+    rng = np.random.Generator(np.random.PCG64(97))
+    for trial in range(300):
+        if trial % 3 == 0:  # whole blocks only
+            la, lb = 16 * int(rng.integers(0, 60)), 16 * int(rng.integers(0, 60))
+            u = max(la, lb, 1) * int(rng.integers(1, 4))
+            a = np.sort(rng.choice(u, size=min(la, u), replace=False)).astype(np.uint32)
+            b = np.sort(rng.choice(u, size=min(lb, u), replace=False)).astype(np.uint32)
+        else:
+            la, lb = rng.integers(0, 1200, size=2)
+            u = int(rng.integers(1, 5000))
+            a, b = sorted_set(rng, la, u), sorted_set(rng, lb, u)
+        exp = np.intersect1d(a, b).astype(np.uint32)
+        assert np.array_equal(oracle.intersect2(algo, a, b), exp), trial
+        assert oracle.count2(algo, a, b) == exp.size
+        assert np.array_equal(oracle.intersect2(scalar, a, b), exp)
+    for trial in range(2000):
+        ma, mb = rng.random(48) < 0.5, rng.random(48) < rng.random()
+        a = np.flatnonzero(ma).astype(np.uint32)
+        b = np.flatnonzero(mb).astype(np.uint32)
+        assert oracle.intersect2(algo, a, b).tolist() == np.flatnonzero(ma & mb).tolist(), trial
+
+
+def test_stream_read_bandwidth_is_measured():
+    gbs = oracle.stream_read_gbs(2, nbytes=64 << 20, reps=2)
+    assert 0.1 < gbs < 10000

@@ -60,8 +60,21 @@ def test_committed_table_covers_the_grid():
 def test_committed_table_choices():
     # the uneven batch is merge work; a 2^24 resident set against sets of 256 is search work
     A, a_off, B, b_off = datagen.uneven_batch(npairs=4096)
-    assert tuning.choose(np.diff(a_off), np.diff(b_off)) == "hip_branchless_merge"
-    assert tuning.choose(np.full(4096, 256), np.full(4096, 1 << 24)) in ("hip_galloping", "hip_binary_search")
+    assert tuning.choose(np.diff(a_off), np.diff(b_off)).partition("@")[0] == "hip_branchless_merge"
+    pick = tuning.choose(np.full(4096, 256), np.full(4096, 1 << 24)).partition("@")[0]
+    assert pick in ("hip_galloping", "hip_binary_search")
+
+
+def test_choose_returns_a_tile_shape():
+    t = _table({(10, 0): {"m": 100.0, "m@1": 120.0, "g": 10.0}, (10, 10): {"m": 10.0, "g": 100.0, "g@2": 90.0}})
+    assert tuning.choose([1024] * 8, [1024] * 8, table=t) == "m@1"
+    assert tuning.choose([1] * 8, [1024] * 8, table=t) == "g"
+
+
+def test_graph_split_threshold_from_the_table():
+    assert tuning.graph_split_threshold(table={"cells": []}) == 32.0
+    t = {"cells": [], "graph_skew_split": {"threshold": 64.0, "ms": {"32.0": 2.0, "64.0": 1.5}}}
+    assert tuning.graph_split_threshold(table=t) == 64.0
 
 
 @pytest.mark.parametrize("larger,smaller,sel", [(64, 64, 0.5), (1024, 8, 0.9), (256, 1, 1.0), (100, 30, 0.0)])
