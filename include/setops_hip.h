@@ -104,6 +104,23 @@ int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a
                               const uint32_t *d_b_states, size_t nb, uint32_t *d_out, uint32_t *d_out_states,
                               size_t out_cap, unsigned long long *d_totals, void *stream);
 
+/* ---- 2-set, device pointers, no allocation -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The same arguments,
+ * contract and results as setops_hip_intersect2_dev, but the call allocates nothing (no hipMalloc,
+ * hipFree or allocating library call): its plan and result staging live in a fixed 20 MiB device
+ * array of the library, one per GPU, held until the process ends.  Limits: na + nb <= 2^22 for the
+ * merge-path entries (branchless_merge, shuffling, broadcast, qfilter and their BSR twins),
+ * min(na, nb) <= 2^20 for the search entries (galloping, binary_search, simd_galloping,
+ * small_adaptive and their BSR twins); larger pairs return SETOPS_ERR_INVALID and enqueue nothing.
+ * Every call on a GPU shares that GPU's array and the array keeps the last call's contents: calls
+ * that overlap on one GPU (on other streams or from other threads) race, so a caller serialises
+ * them (one stream, or a wait between calls). */
+int setops_hip_intersect2_dev_noalloc(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na,
+                                      const uint32_t *d_b, const uint32_t *d_b_states, size_t nb, uint32_t *d_out,
+                                      uint32_t *d_out_states, size_t out_cap, unsigned long long *d_totals,
+                                      void *stream);
+
 /* ---- BSR 2-set, host pointers ------------------------------------------------------------
  * Replaces Intersect2Bsr (setops/src/bsr.rs:16) / UnsafeIntersectBsr
  * (benchmark/src/timer/harness.rs:17) and intersect_qfilter_bsr_b4_v2 (qfilter.h:24-27).

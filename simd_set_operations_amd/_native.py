@@ -69,6 +69,10 @@ HIP_SIGNATURES = {
         _c.c_int,
         [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp],
     ),
+    "setops_hip_intersect2_dev_noalloc": (
+        _c.c_int,
+        [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp],
+    ),
     "setops_hip_intersect2_bsr": (
         _c.c_int,
         [_c.c_int, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp, _vp, _c.c_size_t, _szp, _u64p],

@@ -22,6 +22,13 @@ using namespace setops;
 
 struct setops_hip_batch;
 
+// Device scratch of setops_hip_intersect2_dev_noalloc: the plan and staging of one pair call.  Every
+// GPU the library runs on holds its own copy until the process ends.  On the host this name is only
+// a stand-in for the symbol: host code uses the device address that hipGetSymbolAddress returned
+// for the current device at initialisation (g_scratch), never the name itself.
+constexpr size_t kScratchBytes = 20u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 // ---------------------------------------------------------------------------------------------
@@ -133,6 +140,12 @@ struct Context {
 Context g_ctx;
 std::mutex g_init_mu;
 
+// The scratch array's device address per device (set at initialisation; null until then or if the
+// lookup failed, with the reason in g_scratch_err).
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
+std::string g_scratch_err;
+
 int ensure_init(int device)
 {
     std::lock_guard<std::mutex> lk(g_init_mu);
@@ -153,6 +166,20 @@ int ensure_init(int device)
                     prop.gcnArchName);
     g_ctx.num_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    // the no-allocation entry point's scratch on this device (a failed lookup disables only that
+    // entry point, which then reports the reason)
+    if (device < kMaxDevices) {
+        void *sp = nullptr;
+        const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (se == hipSuccess && sp) {
+            g_scratch[device] = sp;
+        } else {
+            (void)hipGetLastError();
+            g_scratch_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(se);
+        }
+    } else {
+        g_scratch_err = "device index beyond the scratch table";
+    }
     g_ctx.device = device;
     g_ctx.ready = true;
     return SETOPS_OK;
@@ -807,6 +834,82 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     return batch_launch(h, dO, dOs, cap, nullptr, nullptr, dTot, s);
 }
 
+// setops_hip_intersect2_dev_noalloc: one pair planned and staged in the device scratch array.  The
+// launch sequence is single_launch's, enqueued eagerly (no graph capture, no events), and nothing
+// is allocated: the workspace and the staging slots are laid out in the scratch of the current
+// device.  Every pipeline kernel writes what a later one reads (the plan kernel clears the totals,
+// the error word and the counts), so the scratch needs no clearing between calls.
+constexpr uint64_t kNoAllocMaxMerge = 1ull << 22;   // na + nb of a merge-path entry
+constexpr uint64_t kNoAllocMaxProbes = 1ull << 20;  // min(na, nb) of a search entry
+setops_hip_batch g_noalloc;  // the call's host-side description (reused; owns no device memory)
+
+int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, const uint32_t *dB,
+                   const uint32_t *dBs, size_t nb, uint32_t *dO, uint32_t *dOs, size_t cap, unsigned long long *dTot,
+                   hipStream_t s)
+{
+    const AlgoEntry *e = algo_entry(algo);
+    const bool ml = merge_like(e->kind);
+    const uint64_t work = ml ? (uint64_t)na + nb : (uint64_t)std::min(na, nb);
+    const uint64_t limit = ml ? kNoAllocMaxMerge : kNoAllocMaxProbes;
+    if (work > limit)
+        return fail(SETOPS_ERR_INVALID, "%s: %s = %llu exceeds the no-allocation limit of %llu (a %zu-byte scratch)",
+                    e->name, ml ? "na + nb" : "min(na, nb)", (unsigned long long)work, (unsigned long long)limit,
+                    kScratchBytes);
+    char *base = static_cast<char *>(g_scratch[g_ctx.device]);
+    if (!base) return fail(SETOPS_ERR_NO_DEVICE, "no scratch on device %d: %s", g_ctx.device, g_scratch_err.c_str());
+    setops_hip_batch *h = &g_noalloc;
+    h->algo = algo_base(algo);
+    h->shape = shape_of(algo);
+    h->kind = e->kind;
+    h->bsr = e->bsr;
+    h->gather = false;
+    h->P = 1;
+    h->timing = false;
+    h->use_graph = false;
+    h->in = PairBatch{};
+    h->plan = TilePlan{};
+    const uint64_t U = units_per_tile(e->kind, h->shape);
+    const uint64_t units = work + pad_units(e->kind, h->shape);
+    const uint64_t ntiles = std::max<uint64_t>(1, (units + U - 1) / U);
+    const size_t ws = align_up(ws_layout(h, 1, ntiles, base), 256);
+    const size_t stage_bytes = (size_t)ntiles * h->plan.stage_words * 4;
+    if (ws + stage_bytes > kScratchBytes)
+        return fail(SETOPS_ERR_INVALID, "internal: %zu scratch bytes needed, %zu available", ws + stage_bytes,
+                    kScratchBytes);
+    h->plan.stage = reinterpret_cast<uint32_t *>(base + ws);
+    h->plan.utotal = units;
+    h->plan.ntiles = (uint32_t)ntiles;
+    h->plan.c0 = pad_units(e->kind, h->shape);
+    h->plan.U = (uint32_t)U;
+    h->plan.policy = policy_of(e->kind);
+    h->plan.single = 1;
+    h->plan.single_na = na;
+    h->plan.single_nb = nb;
+    h->in.A = dA;
+    h->in.As = dAs;
+    h->in.B = dB;
+    h->in.Bs = dBs;
+    h->in.a_beg = h->plan.single_off;
+    h->in.a_end = h->plan.single_off + 1;
+    h->in.b_beg = h->plan.single_off + 2;
+    h->in.b_end = h->plan.single_off + 3;
+    h->in.a_pool = na;
+    h->in.b_pool = nb;
+    h->in.P = 1;
+    h->in.contig = 1;
+    h->in.sflip = sflip_of(algo);
+    const bool count_only = (dO == nullptr && dOs == nullptr);
+    h->plan.direct = (ntiles == 1 && !e->bsr && (count_only || cap >= std::min(na, nb))) ? 1u : 0u;
+    if (!count_only && h->plan.direct) h->plan.stage = dO;  // the one tile's slot is the output
+    TileOut out{};
+    out.out = dO;
+    out.out_s = dOs;
+    out.cap = count_only ? 0 : cap;
+    out.totals = dTot;
+    out.error = h->d_error;
+    return batch_enqueue(h, out, count_only, s, nullptr);
+}
+
 // One pair, host pointers: upload, run, download.  keys/states of each side.
 int run_single_pair(int algo, const uint32_t *a, const uint32_t *as, size_t na, const uint32_t *b,
                     const uint32_t *bs, size_t nb, uint32_t *out, uint32_t *out_s, size_t out_cap, size_t *out_len,
@@ -1026,6 +1129,19 @@ int setops_hip_intersect2_dev(int algo, const uint32_t *d_a, const uint32_t *d_a
     std::lock_guard<std::mutex> lk(g_ctx.host_mu);
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL = the HIP null stream
     return single_launch(algo, d_a, d_a_states, na, d_b, d_b_states, nb, d_out, d_out_states, out_cap, d_totals, s);
+}
+
+int setops_hip_intersect2_dev_noalloc(int algo, const uint32_t *d_a, const uint32_t *d_a_states, size_t na,
+                                      const uint32_t *d_b, const uint32_t *d_b_states, size_t nb, uint32_t *d_out,
+                                      uint32_t *d_out_states, size_t out_cap, unsigned long long *d_totals,
+                                      void *stream)
+{
+    if (int rc = check_dev_pair(algo, d_a, d_a_states, na, d_b, d_b_states, nb, d_out, d_out_states, d_totals))
+        return rc;
+    ENSURE_INIT();
+    std::lock_guard<std::mutex> lk(g_ctx.host_mu);
+    return noalloc_launch(algo, d_a, d_a_states, na, d_b, d_b_states, nb, d_out, d_out_states, out_cap, d_totals,
+                          static_cast<hipStream_t>(stream));
 }
 
 int setops_hip_intersect2_bsr(int algo, const uint32_t *bases_a, const uint32_t *states_a, size_t na,
