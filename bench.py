@@ -214,10 +214,16 @@ def plain_gpu(torch, A, a_off, B, b_off, sample, stream):
     return {"gelems": E_s / el / 1e9, "pairs": int(len(idx)), "us_per_pair": el / len(idx) * 1e6}
 
 
-def skewed_leg(torch, steps, cpu_threads, with_cpu):
+SKEWED_ENTRIES = ("hip_galloping", "hip_binary_search", "hip_simd_galloping", "hip_galloping:index",
+                  "hip_binary_search:index", "hip_simd_galloping:index", "hip_auto:index")
+
+
+def skewed_leg(torch, steps, cpu_threads, with_cpu, entries=SKEWED_ENTRIES):
     """The skewed workload: one resident large set (2^24 over 2^28) against 4,096 sets of 256 with
     26 planted matches each; the large set is read in place by every pair (gather batch).  Reported
-    on unique bytes: the large set once, the small sets once, outputs and counts once."""
+    on unique bytes: the large set once, the small sets once, outputs and counts once.  `entries`
+    are registered names (``name@shape`` for a tile shape, ``hip_auto`` for the tuning table's
+    choice), each with ``:index`` to search through the B-pool index."""
     from simd_set_operations_amd import datagen
     from simd_set_operations_amd.batch import GatherBatch
 
@@ -236,10 +242,13 @@ def skewed_leg(torch, steps, cpu_threads, with_cpu):
            "elements_streamed": int(P * (L.size + m)), "bytes_unique": B_unique}
     dL = torch.from_numpy(L.view(np.int32)).cuda()
     stream = torch.cuda.current_stream()
-    for algo, indexed in (("hip_galloping", False), ("hip_binary_search", False), ("hip_simd_galloping", False),
-                          ("hip_galloping", True), ("hip_binary_search", True), ("hip_simd_galloping", True)):
+    for entry in entries:
+        algo, _, how = entry.partition(":")
+        indexed = how == "index"
         gb = GatherBatch(algo, pool, a_beg, a_end, dL, b_beg, b_end)
-        key = algo + ("_indexed" if indexed else "")
+        key = (gb.algo if algo == "hip_auto" else algo) + ("_indexed" if indexed else "")
+        if algo == "hip_auto":
+            key = "auto_" + key
         if indexed:
             gb.index_b()  # built once for the resident set, outside the timed region
         reps = steps
@@ -598,6 +607,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=2048)
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-skewed", action="store_true", help="skip the skewed resident-set leg")
+    ap.add_argument("--skewed-only", default="", metavar="ENTRIES",
+                    help="only run the skewed leg over these comma-separated entries (name[@shape][:index]) and "
+                         "print its results as one JSON line: one configuration per process for profiling")
     ap.add_argument("--no-graph", action="store_true", help="skip the R-MAT graph leg")
     ap.add_argument("--graph-scale", type=int, default=22)
     ap.add_argument("--no-serving", action="store_true", help="skip the served k-set leg")
@@ -635,6 +647,11 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     intersect.init(0)
+
+    if args.skewed_only:
+        entries = tuple(e.strip() for e in args.skewed_only.split(",") if e.strip())
+        print(json.dumps({"skewed": skewed_leg(torch, args.steps, 1, False, entries)}, default=float))
+        return
 
     t0 = time.perf_counter()
     A, a_off, B, b_off, shared = datagen.uneven_batch(npairs=args.npairs, with_shared=True)

@@ -76,7 +76,8 @@ constexpr int kCapB = 3072;  // search policy: larger windows staged up to this 
 
 constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this many groups run as one kernel
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
-constexpr int kPlaceQ = 8;  // words per lane per round of the placement kernel
+constexpr int kPlaceNT = 64;   // lanes of a placement workgroup (one tile each)
+constexpr int kPlaceQ = 8;     // words per lane per round of the placement kernel
 
 enum Kind {
     KIND_MERGE = 0,
@@ -295,8 +296,21 @@ uint32_t stage_words(Kind k, bool bsr, int shape)
 // The plan's unit prefix has a closed form only for merge-policy CSR batches (and single pairs).
 bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gather && merge_like(h->kind)); }
 
-// Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields.
-int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
+// Padding units per pair of a batch whose smallest pair owns min_own units.  The padding bounds
+// the pairs one tile touches (<= U / c0 + 2) and keeps every pair at one unit or more.  Search
+// policy: when every pair already owns pad_units of its own, those bounds hold without padding,
+// and the batch has no padding tiles (the skewed workload's 4,096 pairs of 256 probes make
+// exactly 2,048 tiles of 512).  The merge tile needs its padding (a lane's steps past the end of
+// its pair run in the padding), so the merge policy always pads.
+uint32_t pad_for_batch(Kind k, int shape, uint64_t min_own)
+{
+    const uint32_t pad = pad_units(k, shape);
+    return (!merge_like(k) && min_own >= pad) ? 0u : pad;
+}
+
+// Sizes the handle's workspace for P pairs / utotal units (grow-only) and sets the plan fields
+// (c0: padding units per pair, counted in utotal).
+int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal, uint32_t c0)
 {
     const uint64_t U = units_per_tile(h->kind, h->shape);
     const uint64_t ntiles = std::max<uint64_t>(1, (utotal + U - 1) / U);
@@ -316,7 +330,7 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal)
     }
     h->plan.utotal = utotal;
     h->plan.ntiles = (uint32_t)ntiles;
-    h->plan.c0 = pad_units(h->kind, h->shape);
+    h->plan.c0 = c0;
     h->plan.U = (uint32_t)U;
     h->plan.policy = policy_of(h->kind);
     return SETOPS_OK;
@@ -424,7 +438,7 @@ int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t strea
     // up to kScanGroup pairs (serving steps, small gather batches): the scan is one workgroup
     if (ng > 1) {
         hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in,
-                           h->plan);
+                           h->plan, 0u);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, stream, h->plan.gsum,
                            h->plan.gsum2, (uint64_t)ng, 0, h->d_tot);
@@ -438,22 +452,24 @@ int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t strea
 
 // Total units of a batch whose prefix has no closed form: group sums and their scan on the
 // device, total read back (a synchronisation at handle creation only).
-int device_unit_total(setops_hip_batch *h, uint64_t *utotal)
+int device_unit_total(setops_hip_batch *h, uint64_t *utotal, uint64_t *small)
 {
     const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
     hipStream_t s = g_ctx.stream;
     // the offsets may have been written by work still pending on another stream (this count runs
     // on the library's own stream, and creation synchronises anyway)
     HIP_TRY(hipDeviceSynchronize());
-    hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, s, h->in, h->plan);
+    hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, s, h->in, h->plan,
+                       h->plan.c0);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((groups_scan_kernel<kScanGroup>), dim3(1), dim3(kScanGroup), 0, s, h->plan.gsum, h->plan.gsum2,
-                       (uint64_t)ng, 0, h->d_tot);
+                       (uint64_t)ng, 1, h->d_tot);
     HIP_TRY(hipGetLastError());
-    unsigned long long tot = 0;
-    HIP_TRY(hipMemcpyAsync(&tot, h->d_tot, 8, hipMemcpyDeviceToHost, s));
+    unsigned long long tot[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(tot, h->d_tot, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    *utotal = tot;
+    *utotal = tot[0];
+    *small = tot[1];
     return SETOPS_OK;
 }
 
@@ -582,15 +598,29 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
                            direct);
         HIP_TRY(hipGetLastError());
     }
-    // write: one workgroup per group of kPlaceTPW tiles
+    // count-only runs with many tiles: the per-pair offsets alone (one thread per pair)
+    if (place && !fused && count_only) {
+        hipLaunchKernelGGL(pair_offsets_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
+        HIP_TRY(hipGetLastError());
+        return SETOPS_OK;
+    }
+    // write: one workgroup per tile
     if (place) {
-        const uint32_t gp = grid_for((h->plan.ntiles + kPlaceTPW - 1) / kPlaceTPW, 1);
-        if (h->bsr)
-            hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, true>), dim3(gp), dim3(256), 0, stream, h->in, h->plan,
-                               out, fused);
-        else
-            hipLaunchKernelGGL((place_tiles_kernel<256, kPlaceQ, false>), dim3(gp), dim3(256), 0, stream, h->in,
-                               h->plan, out, fused);
+        for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid) {
+            const dim3 gp((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0));
+            if (h->bsr && fused)
+                hipLaunchKernelGGL((place_tiles_kernel<kPlaceNT, kPlaceQ, true, true>), gp, dim3(kPlaceNT), 0, stream,
+                                   h->in, h->plan, out, (uint32_t)t0);
+            else if (h->bsr)
+                hipLaunchKernelGGL((place_tiles_kernel<kPlaceNT, kPlaceQ, true, false>), gp, dim3(kPlaceNT), 0, stream,
+                                   h->in, h->plan, out, (uint32_t)t0);
+            else if (fused)
+                hipLaunchKernelGGL((place_tiles_kernel<kPlaceNT, kPlaceQ, false, true>), gp, dim3(kPlaceNT), 0, stream,
+                                   h->in, h->plan, out, (uint32_t)t0);
+            else
+                hipLaunchKernelGGL((place_tiles_kernel<kPlaceNT, kPlaceQ, false, false>), gp, dim3(kPlaceNT), 0,
+                                   stream, h->in, h->plan, out, (uint32_t)t0);
+        }
         HIP_TRY(hipGetLastError());
     }
     return SETOPS_OK;
@@ -728,14 +758,17 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->plan.single = 0;
     h->plan.direct = 0;
     int rc;
+    const uint32_t pad = pad_units(h->kind, h->shape);
     if (!gather && merge_like(e->kind)) {
-        rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)pad_units(h->kind, h->shape) * npairs);
+        rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)pad * npairs, pad);
     } else {
-        // size the scan arrays first, then count units on the device
-        rc = batch_reserve(h, npairs, 1);
-        uint64_t utotal = 0;
-        if (rc == SETOPS_OK) rc = device_unit_total(h, &utotal);
-        if (rc == SETOPS_OK) rc = batch_reserve(h, npairs, utotal);
+        // size the scan arrays first, then count units (with full padding) and the pairs that own
+        // fewer than `pad` units on the device; without such pairs a search batch needs no padding
+        rc = batch_reserve(h, npairs, 1, pad);
+        uint64_t utotal = 0, small = 0;
+        if (rc == SETOPS_OK) rc = device_unit_total(h, &utotal, &small);
+        const uint32_t c0 = pad_for_batch(h->kind, h->shape, small ? 0 : pad);
+        if (rc == SETOPS_OK) rc = batch_reserve(h, npairs, utotal - (uint64_t)(pad - c0) * npairs, c0);
     }
     if (rc != SETOPS_OK) {
         if (h->ws) (void)hipFree(h->ws);
@@ -810,7 +843,7 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     const uint64_t units =
         (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, h->shape);
     h->plan.single = 1;
-    int rc = batch_reserve(h, 1, units);
+    int rc = batch_reserve(h, 1, units, pad_units(e->kind, h->shape));
     if (rc != SETOPS_OK) return rc;
     h->in.A = dA;
     h->in.As = dAs;
@@ -1311,14 +1344,16 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
     if (!d_a_beg || !d_a_end || !d_b_beg || !d_b_end || !h_na || !h_nb)
         return fail(SETOPS_ERR_INVALID, "NULL argument");
-    const uint32_t c0 = pad_units(handle->kind, handle->shape);
-    uint64_t utotal = 0;
+    uint64_t own = 0, min_own = ~0ull;
     for (uint64_t p = 0; p < npairs; p++) {
         if (h_na[p] >= 0xFFFFFFFFull || h_nb[p] >= 0xFFFFFFFFull)
             return fail(SETOPS_ERR_INVALID, "pair %llu: sets must hold < 2^32 elements", (unsigned long long)p);
-        utotal += (merge_like(handle->kind) ? h_na[p] + h_nb[p] : std::min(h_na[p], h_nb[p])) + c0;
+        const uint64_t u = merge_like(handle->kind) ? h_na[p] + h_nb[p] : std::min(h_na[p], h_nb[p]);
+        own += u;
+        min_own = std::min(min_own, u);
     }
-    int rc = batch_reserve(handle, npairs, utotal);
+    const uint32_t c0 = pad_for_batch(handle->kind, handle->shape, min_own);
+    int rc = batch_reserve(handle, npairs, own + (uint64_t)c0 * npairs, c0);
     if (rc != SETOPS_OK) return rc;
     handle->P = npairs;
     handle->in.P = npairs;

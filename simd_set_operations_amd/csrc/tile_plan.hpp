@@ -26,7 +26,6 @@
 namespace setops {
 
 constexpr uint32_t kScanGroup = 1024;  // elements per group in the three-kernel scans
-constexpr uint32_t kPlaceTPW = 16;     // tiles per workgroup of the placement kernel
 
 struct PairBatch {
     const uint32_t *A;   // A keys (values, or BSR bases)
@@ -200,20 +199,32 @@ __global__ void plan_csr_kernel(PairBatch in, TilePlan plan, TileOut out)
 
 // Group sums: one workgroup per kScanGroup pairs.
 template <int NT>
-__global__ __launch_bounds__(NT) void units_reduce_kernel(PairBatch in, TilePlan plan)
+__global__ __launch_bounds__(NT) void units_reduce_kernel(PairBatch in, TilePlan plan, uint32_t small_below)
 {
     static_assert(NT == (int)kScanGroup, "one thread per element of a group");
-    __shared__ unsigned long long red[NT / kWave];
+    __shared__ unsigned long long red[NT / kWave][2];
     const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
     unsigned long long v = (p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
+    // with small_below: also count the pairs owning fewer than small_below units of their own
+    unsigned long long sm = (small_below && p < in.P && v - plan.c0 < small_below) ? 1ull : 0ull;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) v += __shfl_xor(v, d, kWave);
-    if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+    for (int d = 1; d < kWave; d <<= 1) {
+        v += __shfl_xor(v, d, kWave);
+        sm += __shfl_xor(sm, d, kWave);
+    }
+    if (lane_id() == 0) {
+        red[threadIdx.x / kWave][0] = v;
+        red[threadIdx.x / kWave][1] = sm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long s = 0;
-        for (int w = 0; w < NT / kWave; w++) s += red[w];
+        unsigned long long s = 0, s2 = 0;
+        for (int w = 0; w < NT / kWave; w++) {
+            s += red[w][0];
+            s2 += red[w][1];
+        }
         plan.gsum[blockIdx.x] = s;
+        if (small_below) plan.gsum2[blockIdx.x] = s2;
     }
 }
 
@@ -394,12 +405,13 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
         const uint32_t ngroups = (plan.ntiles + kScanGroup - 1) / kScanGroup;
         const bool last = g + 1 == ngroups;
         const uint32_t end = last ? plan.ntiles : g * kScanGroup;  // the last group sums all tiles
-        // four words per load, four loads in flight per thread
+        // four words per load, LD loads in flight per thread
+        constexpr int LD = 4;
         unsigned long long b = 0, e = 0;
-        for (uint32_t i0 = 4 * threadIdx.x; i0 < end; i0 += 16 * NT) {
-            uint4 c[4], el[4];
+        for (uint32_t i0 = 4 * threadIdx.x; i0 < end; i0 += 4 * LD * NT) {
+            uint4 c[LD], el[LD];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < LD; k++) {
                 const uint32_t i = i0 + 4 * NT * k;
                 if (i < end) {
                     c[k] = *reinterpret_cast<const uint4 *>(plan.tile_count + i);
@@ -407,7 +419,7 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < LD; k++) {
                 const uint32_t i = i0 + 4 * NT * k;
                 if (i < end) {
                     const uint32_t cw[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
@@ -455,148 +467,132 @@ __global__ __launch_bounds__(NT) void scan_tiles_kernel(TilePlan plan, TileOut o
     }
 }
 
+// Count-only runs: the per-pair output offsets alone, one thread per pair.  Pair p's results
+// start in the tile holding its first unit, at the in-tile offset that tile recorded.
+__global__ void pair_offsets_kernel(PairBatch in, TilePlan plan, TileOut out)
+{
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < in.P; p += gsz) {
+        uint64_t t = plan.uoff[p] / plan.U;
+        if (t >= plan.ntiles) t = plan.ntiles - 1;  // stale plans (flagged by the tile kernel): stay inside
+        out.out_off[p] = plan.tile_off[t] + plan.pair_local[p];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
+}
+
 // Write: moves the staged results to their final positions and writes the per-pair output
-// offsets.  One workgroup per group of kPlaceTPW consecutive tiles (grid-stride): their results
-// are one contiguous run of the output, [tile_off[t0], tile_off[t0 + TPW]).  The group's tile
-// offsets (relative to the run's start) go to LDS together with the pair range; then every lane
-// copies words x = lane + NT*j of the run -- it finds the word's tile among the TPW offsets and
-// reads the word from that tile's staging slot -- with all of its loads in flight before its
-// stores, and writes the offsets of the pairs that start in the group (the tile holding a pair's
-// first unit recorded the pair's in-tile result offset).
-// With `fused_scan` (at most kFuseScanTiles tiles) no tile scan ran before: each workgroup sums
-// the tile counts before its group itself, writes its tiles' offsets, and the workgroup of the
-// last group writes the totals, the capacity flag and the end offset.
+// offsets.  One workgroup per tile (tiles tile0 + blockIdx.x): the tile's results are the output
+// run [tile_off[t], tile_off[t+1]); its lanes copy words x = lane + NT*j of the run from the tile's
+// staging slot, with all of a round's loads in flight before its stores, and write the offsets of
+// the pairs whose first unit lies in the tile (the tile recorded each such pair's in-tile result
+// offset).  A workgroup depends on nothing but its own tile's offsets, so the many small
+// workgroups balance tiles of very different result counts; the tile's offsets and pair range
+// are read before anything is written (uniform loads through the scalar cache), and the first
+// pair offsets load before the copy so their round trip overlaps it.
+// FUSED (at most kFuseScanTiles tiles): no tile scan ran before: each workgroup sums the tile
+// counts before its tile itself and writes its tile's offset, and the workgroup of the last tile
+// writes the totals, the capacity flag and the end offsets.
 constexpr uint32_t kFuseScanTiles = 1024;
 
-template <int NT, int Q, bool BSR>
-__global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out, int fused_scan)
+template <int NT, int Q, bool BSR, bool FUSED>
+__global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
-    constexpr uint32_t TPW = kPlaceTPW;
-    static_assert((TPW & (TPW - 1)) == 0 && TPW < (uint32_t)NT, "a log2(TPW)-step search, one thread per tile offset");
-    static_assert(kFuseScanTiles <= 4 * (uint32_t)NT, "four tile counts per thread");
-    __shared__ uint32_t toff[TPW + 1];
-    __shared__ uint32_t gcnt[TPW];
-    __shared__ unsigned long long red[NT / kWave][2];
+    __shared__ unsigned long long red[FUSED ? NT / kWave : 1][2];
     const uint32_t tid = threadIdx.x;
-    const uint64_t ngrp = (plan.ntiles + TPW - 1) / TPW;
+    const uint64_t t = (uint64_t)tile0 + blockIdx.x;
+    if (t >= plan.ntiles) return;
     const uint32_t sw = plan.stage_words;
-    for (uint64_t w = blockIdx.x; w < ngrp; w += gridDim.x) {
-        const uint64_t tw0 = w * TPW;
-        const uint32_t ntw = (plan.ntiles - tw0 < TPW) ? (uint32_t)(plan.ntiles - tw0) : TPW;
-        const bool last_group = w + 1 == ngrp;
-        uint64_t base;
-        if (fused_scan) {
-            // results before the group (and, for the last group, the elements of every tile)
-            unsigned long long b = 0, el = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t i = tid + (uint32_t)k * NT;
-                if (i < tw0) b += plan.tile_count[i];
-                if (last_group && i < plan.ntiles) el += plan.tile_elems[i];
-            }
-            if (tid < ntw) gcnt[tid] = plan.tile_count[tw0 + tid];
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                b += __shfl_xor(b, d, kWave);
-                el += __shfl_xor(el, d, kWave);
-            }
-            if (lane_id() == 0) {
-                red[tid / kWave][0] = b;
-                red[tid / kWave][1] = el;
-            }
-            __syncthreads();
-            base = 0;
-            el = 0;
-            for (int k = 0; k < NT / kWave; k++) {
-                base += red[k][0];
-                el += red[k][1];
-            }
-            if (tid <= TPW) {
-                uint32_t o = 0;
-                for (uint32_t j = 0; j < tid && j < ntw; j++) o += gcnt[j];
-                toff[tid] = (tid <= ntw) ? o : ~0u;
-                if (tid < ntw) plan.tile_off[tw0 + tid] = base + o;
-                if (last_group && tid == ntw) {
-                    const unsigned long long total = base + o;
-                    plan.tile_off[plan.ntiles] = total;
-                    if (out.totals) {
-                        out.totals[0] = total;
-                        out.totals[1] = el;
-                    }
-                    if (out.out && total > out.cap && out.error) atomicOr(out.error, 1u);
-                    if (out.out_off) out.out_off[in.P] = total;
-                }
-            }
-        } else {
-            base = plan.tile_off[tw0];
-            // toff[k]: tile k's first word, relative to the group's (< TPW * stage slot); past the
-            // group's end: all ones, so the search never stops there
-            if (tid <= TPW) toff[tid] = (tid <= ntw) ? (uint32_t)(plan.tile_off[tw0 + tid] - base) : ~0u;
+    const uint64_t pf = plan.tile_first[t];
+    uint64_t pl = plan.tile_last[t];
+    if (pl >= in.P) pl = in.P - 1;  // stale maps (flagged by the tile kernel): stay inside the pairs
+    uint64_t base;
+    uint32_t n;
+    if constexpr (FUSED) {
+        // results before the tile (and, for the last tile, the elements of every tile)
+        const bool last = t + 1 == plan.ntiles;
+        unsigned long long b = 0, el = 0;
+        for (uint32_t i = tid; i < plan.ntiles; i += NT) {
+            if (i < t) b += plan.tile_count[i];
+            if (last) el += plan.tile_elems[i];
         }
-        const uint64_t pf = plan.tile_first[tw0];
-        uint64_t pl = plan.tile_last[tw0 + ntw - 1];
-        if (pl >= in.P) pl = in.P - 1;  // stale maps (flagged by the tile kernel): stay inside the pairs
-        // the first pairs' prefix entries and in-tile offsets load with the tile offsets
-        const uint64_t p = pf + tid;
-        const bool pv = out.out_off && p <= pl;
-        uint64_t pu = 0;
-        uint32_t ploc = 0;
-        if (pv) {
-            pu = plan.uoff[p];
-            ploc = plan.pair_local[p];
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            b += __shfl_xor(b, d, kWave);
+            el += __shfl_xor(el, d, kWave);
+        }
+        if (lane_id() == 0) {
+            red[tid / kWave][0] = b;
+            red[tid / kWave][1] = el;
         }
         __syncthreads();
-        const uint32_t nw = toff[ntw];
-        if (out.out) {
-            const uint64_t room = out.cap > base ? out.cap - base : 0;
-            const uint32_t lim = (uint64_t)nw < room ? nw : (uint32_t)room;
-            // 32-bit byte offsets from uniform bases (one address register per word)
-            const char *stage = reinterpret_cast<const char *>(plan.stage + tw0 * sw);
-            char *dk = reinterpret_cast<char *>(out.out + base);
-            char *ds = BSR ? reinterpret_cast<char *>(out.out_s + base) : nullptr;
-            const uint32_t sbytes = 2u * sw;  // byte distance from a key to its state (sw/2 words)
-            for (uint32_t x0 = 0; x0 < lim; x0 += (uint32_t)NT * Q) {
-                uint32_t boff[Q];  // staging byte offset (from the group's first slot) of word x
-#pragma unroll
-                for (int j = 0; j < Q; j++) {
-                    const uint32_t x = x0 + tid + (uint32_t)j * NT;
-                    // the largest lo < TPW with toff[lo] <= x, in log2(TPW) fixed steps (the words'
-                    // searches are independent, so their LDS reads overlap)
-                    uint32_t lo = 0;
-#pragma unroll
-                    for (uint32_t st = TPW / 2; st >= 1; st >>= 1) lo += (toff[lo + st] <= x) ? st : 0u;
-                    boff[j] = 4u * (lo * sw + (x - toff[lo]));
+        base = 0;
+        el = 0;
+        for (int k = 0; k < NT / kWave; k++) {
+            base += red[k][0];
+            el += red[k][1];
+        }
+        n = plan.tile_count[t];
+        if (tid == 0) {
+            plan.tile_off[t] = base;
+            if (last) {
+                const unsigned long long total = base + n;
+                plan.tile_off[plan.ntiles] = total;
+                if (out.totals) {
+                    out.totals[0] = total;
+                    out.totals[1] = el;
                 }
-                uint32_t v[Q], s2[BSR ? Q : 1];
-#pragma unroll
-                for (int j = 0; j < Q; j++) {
-                    if (x0 + tid + (uint32_t)j * NT < lim) {
-                        v[j] = *reinterpret_cast<const uint32_t *>(stage + boff[j]);
-                        if constexpr (BSR) s2[j] = *reinterpret_cast<const uint32_t *>(stage + (boff[j] + sbytes));
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < Q; j++) {
-                    const uint32_t x = x0 + tid + (uint32_t)j * NT;
-                    if (x < lim) {
-                        *reinterpret_cast<uint32_t *>(dk + 4u * x) = v[j];
-                        if constexpr (BSR) *reinterpret_cast<uint32_t *>(ds + 4u * x) = s2[j];
-                    }
-                }
+                if (out.out && total > out.cap && out.error) atomicOr(out.error, 1u);
+                if (out.out_off) out.out_off[in.P] = total;
             }
         }
-        if (out.out_off) {
-            const uint64_t U = plan.U, u0 = tw0 * U;
-            // a pair whose first unit lies before the group started in an earlier group
-            if (pv && pu >= u0) out.out_off[p] = base + toff[(uint32_t)(pu - u0) / (uint32_t)U] + ploc;
-            for (uint64_t q = p + NT; q <= pl; q += NT) {  // groups of more than NT pairs
-                const uint64_t uq = plan.uoff[q];
-                if (uq >= u0) out.out_off[q] = base + toff[(uint32_t)(uq - u0) / (uint32_t)U] + plan.pair_local[q];
-            }
-        }
-        __syncthreads();  // toff (and red, gcnt) are rewritten by the next group
+    } else {
+        base = plan.tile_off[t];
+        n = (uint32_t)(plan.tile_off[t + 1] - base);
+        if (out.out_off && t == 0 && tid == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
     }
-    if (!fused_scan && out.out_off && blockIdx.x == 0 && tid == 0) out.out_off[in.P] = plan.tile_off[plan.ntiles];
+    // the first pair offsets (a pair whose first unit lies in an earlier tile got its offset there)
+    const uint64_t u0 = t * plan.U;
+    const uint64_t p0 = pf + tid;
+    uint64_t up = 0;
+    uint32_t ploc = 0;
+    if (out.out_off && p0 <= pl) {
+        up = plan.uoff[p0];
+        ploc = plan.pair_local[p0];
+    }
+    if (out.out) {
+        const uint64_t room = out.cap > base ? out.cap - base : 0;
+        const uint32_t lim = (uint64_t)n < room ? n : (uint32_t)room;
+        const uint32_t *src = plan.stage + t * sw;
+        uint32_t *dk = out.out + base;
+        uint32_t *ds = BSR ? out.out_s + base : nullptr;
+        for (uint32_t x0 = 0; x0 < lim; x0 += (uint32_t)NT * Q) {
+            uint32_t v[Q], s2[BSR ? Q : 1];
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const uint32_t x = x0 + tid + (uint32_t)j * NT;
+                if (x < lim) {
+                    v[j] = src[x];
+                    if constexpr (BSR) s2[j] = src[sw / 2 + x];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const uint32_t x = x0 + tid + (uint32_t)j * NT;
+                if (x < lim) {
+                    dk[x] = v[j];
+                    if constexpr (BSR) ds[x] = s2[j];
+                }
+            }
+        }
+    }
+    if (out.out_off) {
+        if (p0 <= pl && up >= u0) out.out_off[p0] = base + ploc;
+        for (uint64_t p = p0 + NT; p <= pl; p += NT) {  // tiles of more than NT pairs
+            const uint64_t uq = plan.uoff[p];
+            const uint32_t lq = plan.pair_local[p];
+            if (uq >= u0) out.out_off[p] = base + lq;
+        }
+    }
 }
 
 // ---- per-tile segment geometry -------------------------------------------------------------------

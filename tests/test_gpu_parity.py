@@ -449,6 +449,68 @@ def test_indexed_b_pool_ragged_ranges(name):
             assert np.array_equal(res[0][2][g_off[p]:g_off[p + 1]], exp), p
 
 
+@pytest.mark.parametrize("name", ["hip_galloping", "hip_binary_search@1", "hip_simd_galloping", "hip_galloping@2",
+                                  "hip_galloping_bsr"])
+def test_search_batches_without_padding(name):
+    """Search-policy gather batches whose every pair has at least 2*KS probes (ceil(U/128)) carry
+    no padding units: exactly ceil(sum of probes / U) tiles, up to 128 pairs in one tile (pairs of
+    the minimum size), pair boundaries anywhere inside lanes.  One pair below the minimum brings
+    the padding back.  Results equal the oracle's for every pair, on two runs."""
+    from simd_set_operations_amd.batch import GatherBatch
+
+    bsr = name.endswith("_bsr")
+    ks = {"@1": 1, "@2": 4}.get(name[-2:], 2)
+    U, pad = 256 * ks, 2 * ks
+    rng = np.random.Generator(np.random.PCG64(151))
+    large = sorted_set(rng, 1 << 18, 1 << 24)
+    for small_pair in (False, True):
+        # the smaller side of each pair: the minimum size for runs of pairs (128 per tile), then
+        # random sizes; half of each small set drawn from the large set
+        sizes = [pad] * 700 + [int(rng.integers(pad, 3 * U)) for _ in range(600)]
+        if small_pair:
+            sizes[333] = pad - 1
+        smalls = []
+        for m in sizes:
+            hit = rng.choice(large, size=m // 2, replace=False)
+            s_ = np.union1d(hit, rng.integers(0, 1 << 24, size=m - m // 2)).astype(np.uint32)
+            while s_.size < m:  # collisions: top up to the exact size
+                s_ = np.union1d(s_, rng.integers(0, 1 << 24, size=m - s_.size)).astype(np.uint32)
+            smalls.append(s_)
+        P = len(smalls)
+        pool, off = _csr(smalls)
+        probes = int(np.diff(off).sum())
+        kw = {}
+        if bsr:
+            lb = BsrVec.from_sorted(large)
+            sb = [BsrVec.from_sorted(x) for x in smalls]
+            pool, off = _csr([b.bases for b in sb])
+            spool, _ = _csr([b.states for b in sb])
+            probes = int(np.minimum(np.diff(off), lb.bases.size).sum())
+            kw = {"A_states": spool, "B_states": lb.states}
+            L = lb.bases
+        else:
+            L = large
+        announce(f"{name} unpadded={not small_pair} P={P} probes={probes}")
+        gb = GatherBatch(name, pool, off[:-1], off[1:], L, np.zeros(P, np.uint64), np.full(P, L.size, np.uint64),
+                         **kw)
+        unpadded = not small_pair and int(np.diff(off).min()) >= pad
+        expect_tiles = -(-(probes + (0 if unpadded else pad * P)) // U)
+        assert gb.info()["tiles"] == expect_tiles
+        if bsr:
+            expect = [oracle.bsr_intersect("branchless_merge_bsr", pool[off[p]:off[p + 1]], spool[off[p]:off[p + 1]],
+                                           lb.bases, lb.states) for p in range(P)]
+        else:
+            expect = [(oracle.branchless_merge(smalls[p], large),) for p in range(P)]
+        for rep in range(2):
+            gb.run()
+            res = gb.results()
+            g_off = res[0]
+            for p in range(P):
+                for j, e in enumerate(expect[p]):
+                    assert np.array_equal(res[2 + j][g_off[p]:g_off[p + 1]], e), (rep, p, j)
+        gb.close()
+
+
 # ---- graph edges (count-only gather batches) ------------------------------------------------------
 def test_graph_triangle_sums():
     """R-MAT edges: per-edge counts of every entry equal the oracle's; full-list sums are three times
