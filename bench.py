@@ -43,7 +43,7 @@ def log(msg):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "merge_tile_kernel_uneven_batch_traffic.txt")
-TRACE_FILE = os.path.join(ROOT, "profiles", "uneven_batch_and_skewed_kernel_trace.txt")
+TRACE_FILE = os.path.join(ROOT, "profiles", "uneven_batch_kernel_trace.txt")
 # Host threads for the CPU baselines: the host cores this process may run on, at most 16 (the
 # host-CPU share a GPU of this pool comes with, whatever the machine's total core count is).
 CPU_SHARE = 16
