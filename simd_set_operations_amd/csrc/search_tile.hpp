@@ -197,72 +197,152 @@ __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, 
     return hit;
 }
 
-// index_probe with a top level of the index in LDS: top[j - tj0] = idx[kTopIdx j] (= B[kTopStride j])
-// for every j whose kTopIdx*j lies in the probe's entry range, and a middle level in global memory,
-// mid[m] = idx[kMidIdx m].  The LDS search leaves kTopIdx index entries; the middle level's 16
-// entries over them, then the 16 index entries they leave, each load in one round trip; then one
-// 128-byte line of B, as in index_probe.  Three dependent global round trips per probe.
-// Entries <= x among arr[lo, hi) of a sorted array of n entries, hi - lo <= 16: the aligned 16-byte
-// chunks that cover the range load together (one round trip).
-__device__ __forceinline__ uint32_t count_le(const uint32_t *__restrict__ arr, uint64_t n, uint64_t lo, uint64_t hi,
-                                             uint32_t x)
+// ---- wave-cooperative index lookups --------------------------------------------------------------
+// A lookup through the B-pool index with its top level in LDS (every kTopStride-th element) searches
+// the top level in LDS, then makes three dependent global reads: a 64-byte window of the middle
+// level (every kMidStride-th element), a 64-byte window of the index (every 32nd element) and the
+// 128-byte line of B that can hold x.  Read by the probe's own lane, each window is four or eight
+// 16-byte loads to one line, and each load instruction of a wave touches 64 different lines: the
+// vector memory pipeline looks every one of them up separately, and on resident pools that, not
+// HBM, bounds the search tile.  Here a group of lanes reads one probe's window, one 16-byte chunk
+// per lane, so an instruction reads 16 whole 64-byte windows (groups of 4) or 8 whole 128-byte lines
+// (groups of 8); the probes' arguments reach the groups, and the counts come back, through
+// cross-lane permutes.  Every lane of the wave must call these together.
+
+__device__ __forceinline__ uint32_t lane_perm(uint32_t v, uint32_t src)
 {
-    uint32_t cnt = 0;
-    const uint64_t c0 = lo & ~3ull;
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-        const uint64_t e = c0 + 4ull * (uint64_t)c;
-        if (e < hi) {
-            const uint4 v = load_chunk(arr, n, e);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint64_t q = e + (uint64_t)j;
-                cnt += (q >= lo && q < hi && w[j] <= x) ? 1u : 0u;
-            }
-        }
-    }
-    return cnt;
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
 
-__device__ __forceinline__ uint64_t index_probe_top(const uint32_t *__restrict__ B, uint64_t pool,
-                                                    const uint32_t *__restrict__ idx,
-                                                    const uint32_t *__restrict__ mid, const uint32_t *top,
-                                                    uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x, uint32_t f)
+// For the probe of every lane: entries <= x among positions [lo, hi) of the 16-entry block blk of
+// arr (entries 16 blk .. 16 blk + 15, arr holding n entries); lh = lo | hi << 8, 0 for no window.
+__device__ __forceinline__ uint32_t coop_count16(const uint32_t *__restrict__ arr, uint64_t n, uint32_t blk,
+                                                 uint32_t lh, uint32_t x)
 {
-    if (g0 >= g1) return kNoPos;
-    const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;                // entries with 32k in [g0, g1)
-    const uint64_t jb = (kb + kTopIdx - 1) / kTopIdx, je = (ke + kTopIdx - 1) / kTopIdx;  // top entries inside
-    uint64_t lo = jb, hi = je;  // first top entry > x
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (top[mid - tj0] <= x) lo = mid + 1; else hi = mid;
+    const uint32_t lane = lane_id();
+    const uint32_t c = lane & 3u;  // the chunk this lane reads of the block it serves
+    uint4 v[4];
+    uint32_t plh[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t p = 16u * (uint32_t)r + (lane >> 2);  // the probe served in round r
+        const uint32_t pb = lane_perm(blk, p);
+        plh[r] = lane_perm(lh, p);
+        v[r] = plh[r] ? load_chunk(arr, n, 16ull * pb + 4u * c) : make_uint4(0u, 0u, 0u, 0u);
     }
-    // the first index entry > x lies in [L, H]: idx[kTopIdx (j-1)] <= x < idx[kTopIdx j] for j = lo
-    const uint64_t L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
-    const uint64_t H = (lo < je) ? kTopIdx * lo : ke;
-    // middle level mid[m] = idx[kMidIdx m]: the at most kTopIdx / kMidIdx entries with kMidIdx m in
-    // [L, H) load together and are counted; then the at most kMidIdx index entries they leave
-    const uint64_t mb = (L + kMidIdx - 1) / kMidIdx, me = (H + kMidIdx - 1) / kMidIdx;
-    const uint64_t ms = mb + count_le(mid, (pool + kMidStride - 1) / kMidStride, mb, me, x);
-    const uint64_t L2 = (ms > mb) ? kMidIdx * (ms - 1) + 1 : L;
-    const uint64_t H2 = (ms < me) ? kMidIdx * ms : H;
-    const uint64_t k = L2 + count_le(idx, (pool + 31) / 32, L2, H2, x);  // first index entry > x
-    if (k == 0) return kNoPos;  // x < B[0]
-    const uint64_t base = 32 * (k - 1);  // the line [base, base+32) holds every candidate
-    uint64_t hit = kNoPos;
+    uint32_t res = 0;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const uint64_t e = base + 4ull * c;
-        const uint4 v = load_chunk(B, pool, e);
-        const uint32_t w[4] = {v.x ^ f, v.y ^ f, v.z ^ f, v.w ^ f};
+    for (int r = 0; r < 4; r++) {
+        const uint32_t p = 16u * (uint32_t)r + (lane >> 2);
+        const uint32_t px = lane_perm(x, p);
+        const uint32_t lo = plh[r] & 0xFFu, hi = plh[r] >> 8;
+        const uint32_t w[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+        uint32_t k = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t q = e + (uint64_t)j;
-            if (q >= g0 && q < g1 && q < pool && w[j] == x) hit = q;
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t e = 4u * c + j;
+            k += (e >= lo && e < hi && w[j] <= px) ? 1u : 0u;
         }
+        k += __shfl_xor(k, 1, kWave);
+        k += __shfl_xor(k, 2, kWave);
+        const uint32_t t = lane_perm(k, 4u * (lane & 15u));  // probe lane 16r + i reads group i
+        if ((lane >> 4) == (uint32_t)r) res = t;
     }
-    return hit;
+    return res;
+}
+
+// For the probe of every lane: the position of x (compared XOR f) among positions [lo, hi) of the
+// 32-element line `line` of B (elements 32 line .. 32 line + 31), or 32 when absent; lh as above.
+__device__ __forceinline__ uint32_t coop_find32(const uint32_t *__restrict__ B, uint64_t pool, uint32_t line,
+                                                uint32_t lh, uint32_t x, uint32_t f)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t c = lane & 7u;
+    uint4 v[8];
+    uint32_t plh[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const uint32_t p = 8u * (uint32_t)r + (lane >> 3);
+        const uint32_t pl = lane_perm(line, p);
+        plh[r] = lane_perm(lh, p);
+        v[r] = plh[r] ? load_chunk(B, pool, 32ull * pl + 4u * c) : make_uint4(0u, 0u, 0u, 0u);
+    }
+    uint32_t res = 32;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const uint32_t p = 8u * (uint32_t)r + (lane >> 3);
+        const uint32_t px = lane_perm(x, p);
+        const uint32_t lo = plh[r] & 0xFFu, hi = plh[r] >> 8;
+        const uint32_t w[4] = {v[r].x ^ f, v[r].y ^ f, v[r].z ^ f, v[r].w ^ f};
+        uint32_t h = 0;  // position + 1 of the match (a set holds x at most once)
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t e = 4u * c + j;
+            h = (e >= lo && e < hi && w[j] == px) ? e + 1u : h;
+        }
+        h |= __shfl_xor(h, 1, kWave);
+        h |= __shfl_xor(h, 2, kWave);
+        h |= __shfl_xor(h, 4, kWave);
+        const uint32_t t = lane_perm(h, 8u * (lane & 7u));  // probe lane 8r + i reads group i
+        if ((lane >> 3) == (uint32_t)r) res = t ? t - 1u : 32u;
+    }
+    return res;
+}
+
+// Pool index of x in B[g0, g1), or kNoPos, for the probe of every lane of the wave (on = false: no
+// probe), through the top level top[j - tj0] = B[kTopStride j] in LDS, the middle level
+// mid[m] = idx[kMidIdx m] and the index idx[k] = B[32 k] (values in the compare space), with the
+// global reads made by groups of lanes.  The windows it reads each lie in one aligned block:
+// the middle-level entries over one top-level interval are those of one 16-entry block (top
+// entries are every 16th middle entry), likewise the index entries under one middle-level
+// interval, and the candidates of B one 32-element line.
+__device__ __forceinline__ uint64_t coop_index_probe_top(bool on, const uint32_t *__restrict__ B, uint64_t pool,
+                                                         const uint32_t *__restrict__ idx,
+                                                         const uint32_t *__restrict__ mid, const uint32_t *top,
+                                                         uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x, uint32_t f)
+{
+    on = on && g0 < g1;
+    uint64_t L = 0, H = 0, mb = 0, me = 0;
+    if (on) {
+        const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;
+        const uint64_t jb = (kb + kTopIdx - 1) / kTopIdx, je = (ke + kTopIdx - 1) / kTopIdx;
+        uint64_t lo = jb, hi = je;  // first top entry > x
+        while (lo < hi) {
+            const uint64_t m = (lo + hi) >> 1;
+            if (top[m - tj0] <= x) lo = m + 1; else hi = m;
+        }
+        L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
+        H = (lo < je) ? kTopIdx * lo : ke;
+        mb = (L + kMidIdx - 1) / kMidIdx;
+        me = (H + kMidIdx - 1) / kMidIdx;
+    }
+    // middle level: [mb, me) inside block mb / 16
+    const uint32_t mblk = (uint32_t)(mb >> 4);
+    const uint32_t mlh = (on && me > mb) ? (uint32_t)(mb & 15u) | ((uint32_t)(me - 16ull * mblk) << 8) : 0u;
+    const uint32_t mc = coop_count16(mid, (pool + kMidStride - 1) / kMidStride, mblk, mlh, x);
+    uint64_t L2 = 0, H2 = 0;
+    if (on) {
+        const uint64_t ms = mb + mc;
+        L2 = (ms > mb) ? kMidIdx * (ms - 1) + 1 : L;
+        H2 = (ms < me) ? kMidIdx * ms : H;
+    }
+    // index: [L2, H2) inside block L2 / 16; the first entry > x is L2 + count
+    const uint32_t iblk = (uint32_t)(L2 >> 4);
+    const uint32_t ilh = (on && H2 > L2) ? (uint32_t)(L2 & 15u) | ((uint32_t)(H2 - 16ull * iblk) << 8) : 0u;
+    const uint32_t ic = coop_count16(idx, (pool + 31) / 32, iblk, ilh, x);
+    const uint64_t k = L2 + ic;
+    // B: the line [32(k-1), 32k) holds every candidate; positions inside [g0, g1) and the pool
+    const bool bon = on && k > 0;
+    const uint64_t base = bon ? 32 * (k - 1) : 0;
+    uint32_t blh = 0;
+    if (bon) {
+        const uint64_t e1 = g1 < pool ? g1 : pool;
+        const uint32_t lo = g0 > base ? (uint32_t)min(g0 - base, (uint64_t)32) : 0u;
+        const uint32_t hi = e1 > base ? (uint32_t)min(e1 - base, (uint64_t)32) : 0u;
+        blh = lo < hi ? lo | (hi << 8) : 0u;
+    }
+    const uint32_t pos = coop_find32(B, pool, (uint32_t)(base >> 5), blh, x, f);
+    return (bon && pos < 32) ? base + pos : kNoPos;
 }
 
 template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, int SMODE>
@@ -480,8 +560,45 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                 if (sm.seg_u[mid] <= ub) lo = mid; else hi = mid;
             }
             s0 = lo;
+        }
+        // 4a. probes answered through the indexed B pool with its top level staged: looked up
+        // first, the whole wave together (coop_index_probe_top), the k-th probe of every lane at
+        // once; qrel[k] is the found position relative to the larger set, or ~0.  A probe's range
+        // starts at its narrowed window rather than at the previous hit: a set holds x once, so the
+        // position found is the same.
+        uint32_t qrel[KS];
+#pragma unroll
+        for (int k = 0; k < KS; k++) qrel[k] = ~0u;
+        if (top_ok && in.b_idx) {
+            uint32_t s = s0;
+#pragma unroll
+            for (int k = 0; k < KS; k++) {
+                const uint32_t u = ub + (uint32_t)k;
+                bool on = u < ulen;
+                uint32_t x = 0;
+                uint64_t g = 0, g0 = 0, g1 = 0;
+                if (on) {
+                    while (u >= sm.seg_u[s + 1]) s++;
+                    const uint32_t e = u - sm.seg_u[s];
+                    on = e < sm.seg_wa[s] && sm.seg_lb[s] == kNotStaged && !((sm.seg_flags[s] >> 2) & 1u);
+                    if (on) {
+                        x = sm.pkey[sm.seg_la[s] + e];
+                        g = sm.seg_lbase[s];
+                        g0 = g + sm.seg_lo[s];
+                        g1 = g + sm.seg_hi[s];
+                    }
+                }
+                if (ballot(on) != 0) {  // wave-uniform
+                    const uint64_t q = coop_index_probe_top(on, in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g0, g1,
+                                                            x, in.sflip);
+                    if (q != kNoPos) qrel[k] = (uint32_t)(q - g);
+                }
+            }
+        }
+        if (ub < ulen) {
             uint32_t s = s0;
             uint32_t cur = sm.seg_lo[s];  // pair-relative position in the larger set
+#pragma unroll
             for (int k = 0; k < KS; k++) {
                 const uint32_t u = ub + (uint32_t)k;
                 if (u >= ulen) break;
@@ -506,10 +623,9 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
                         if (found) lstate = sm.lst[lb + (pos - lo_s)];
                 } else if (in.b_idx && !large_is_a) {
                     // resident B pool with an index of every 32nd element: binary search the index,
-                    // then compare one 128-byte line of B
+                    // then compare one 128-byte line of B (with the top level staged: looked up in 4a)
                     const uint64_t g = sm.seg_lbase[s];
-                    const uint64_t q = top_ok ? index_probe_top(in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g + cur,
-                                                                g + hi_s, x, in.sflip)
+                    const uint64_t q = top_ok ? (qrel[k] != ~0u ? g + qrel[k] : kNoPos)
                                               : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x, in.sflip);
                     found = q != kNoPos;
                     pos = found ? (uint32_t)(q - g) : cur;
