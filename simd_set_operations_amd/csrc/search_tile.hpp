@@ -214,27 +214,32 @@ __device__ __forceinline__ uint32_t lane_perm(uint32_t v, uint32_t src)
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
 
-// For the probe of every lane: entries <= x among positions [lo, hi) of the 16-entry block blk of
-// arr (entries 16 blk .. 16 blk + 15, arr holding n entries); lh = lo | hi << 8, 0 for no window.
-__device__ __forceinline__ uint32_t coop_count16(const uint32_t *__restrict__ arr, uint64_t n, uint32_t blk,
-                                                 uint32_t lh, uint32_t x)
+// For the NP probes of every lane (probe k of lane p: window blk[k], lh[k], x[k]): entries <= x among
+// positions [lo, hi) of the 16-entry block blk of arr (entries 16 blk .. 16 blk + 15, arr holding
+// n entries); lh = lo | hi << 8, 0 for no window.  Round r serves probe k = r / 4 of lanes
+// 16 (r % 4) .. 16 (r % 4) + 15, one per group of 4 lanes; all 4 NP loads are in flight together.
+template <int NP>
+__device__ __forceinline__ void coop_count16(const uint32_t *__restrict__ arr, uint64_t n, const uint32_t (&blk)[NP],
+                                             const uint32_t (&lh)[NP], const uint32_t (&x)[NP], uint32_t (&res)[NP])
 {
+    constexpr int R = 4 * NP;
     const uint32_t lane = lane_id();
     const uint32_t c = lane & 3u;  // the chunk this lane reads of the block it serves
-    uint4 v[4];
-    uint32_t plh[4];
+    uint4 v[R];
+    uint32_t plh[R];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const uint32_t p = 16u * (uint32_t)r + (lane >> 2);  // the probe served in round r
-        const uint32_t pb = lane_perm(blk, p);
-        plh[r] = lane_perm(lh, p);
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = 16u * (uint32_t)(r & 3) + (lane >> 2);  // the lane whose probe r / 4 is served
+        const uint32_t pb = lane_perm(blk[r >> 2], p);
+        plh[r] = lane_perm(lh[r >> 2], p);
         v[r] = plh[r] ? load_chunk(arr, n, 16ull * pb + 4u * c) : make_uint4(0u, 0u, 0u, 0u);
     }
-    uint32_t res = 0;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const uint32_t p = 16u * (uint32_t)r + (lane >> 2);
-        const uint32_t px = lane_perm(x, p);
+    for (int k = 0; k < NP; k++) res[k] = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = 16u * (uint32_t)(r & 3) + (lane >> 2);
+        const uint32_t px = lane_perm(x[r >> 2], p);
         const uint32_t lo = plh[r] & 0xFFu, hi = plh[r] >> 8;
         const uint32_t w[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
         uint32_t k = 0;
@@ -245,33 +250,37 @@ __device__ __forceinline__ uint32_t coop_count16(const uint32_t *__restrict__ ar
         }
         k += __shfl_xor(k, 1, kWave);
         k += __shfl_xor(k, 2, kWave);
-        const uint32_t t = lane_perm(k, 4u * (lane & 15u));  // probe lane 16r + i reads group i
-        if ((lane >> 4) == (uint32_t)r) res = t;
+        const uint32_t t = lane_perm(k, 4u * (lane & 15u));  // lane 16 (r % 4) + i reads group i
+        if ((lane >> 4) == (uint32_t)(r & 3)) res[r >> 2] = t;
     }
-    return res;
 }
 
-// For the probe of every lane: the position of x (compared XOR f) among positions [lo, hi) of the
-// 32-element line `line` of B (elements 32 line .. 32 line + 31), or 32 when absent; lh as above.
-__device__ __forceinline__ uint32_t coop_find32(const uint32_t *__restrict__ B, uint64_t pool, uint32_t line,
-                                                uint32_t lh, uint32_t x, uint32_t f)
+// For the NP probes of every lane: the position of x (compared XOR f) among positions [lo, hi) of
+// the 32-element line `line` of B (elements 32 line .. 32 line + 31), or 32 when absent; lh as
+// above.  Round r serves probe k = r / 8 of lanes 8 (r % 8) .. 8 (r % 8) + 7, one per group of 8.
+template <int NP>
+__device__ __forceinline__ void coop_find32(const uint32_t *__restrict__ B, uint64_t pool, const uint32_t (&line)[NP],
+                                            const uint32_t (&lh)[NP], const uint32_t (&x)[NP], uint32_t f,
+                                            uint32_t (&res)[NP])
 {
+    constexpr int R = 8 * NP;
     const uint32_t lane = lane_id();
     const uint32_t c = lane & 7u;
-    uint4 v[8];
-    uint32_t plh[8];
+    uint4 v[R];
+    uint32_t plh[R];
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const uint32_t p = 8u * (uint32_t)r + (lane >> 3);
-        const uint32_t pl = lane_perm(line, p);
-        plh[r] = lane_perm(lh, p);
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = 8u * (uint32_t)(r & 7) + (lane >> 3);
+        const uint32_t pl = lane_perm(line[r >> 3], p);
+        plh[r] = lane_perm(lh[r >> 3], p);
         v[r] = plh[r] ? load_chunk(B, pool, 32ull * pl + 4u * c) : make_uint4(0u, 0u, 0u, 0u);
     }
-    uint32_t res = 32;
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const uint32_t p = 8u * (uint32_t)r + (lane >> 3);
-        const uint32_t px = lane_perm(x, p);
+    for (int k = 0; k < NP; k++) res[k] = 32;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = 8u * (uint32_t)(r & 7) + (lane >> 3);
+        const uint32_t px = lane_perm(x[r >> 3], p);
         const uint32_t lo = plh[r] & 0xFFu, hi = plh[r] >> 8;
         const uint32_t w[4] = {v[r].x ^ f, v[r].y ^ f, v[r].z ^ f, v[r].w ^ f};
         uint32_t h = 0;  // position + 1 of the match (a set holds x at most once)
@@ -283,70 +292,93 @@ __device__ __forceinline__ uint32_t coop_find32(const uint32_t *__restrict__ B, 
         h |= __shfl_xor(h, 1, kWave);
         h |= __shfl_xor(h, 2, kWave);
         h |= __shfl_xor(h, 4, kWave);
-        const uint32_t t = lane_perm(h, 8u * (lane & 7u));  // probe lane 8r + i reads group i
-        if ((lane >> 3) == (uint32_t)r) res = t ? t - 1u : 32u;
+        const uint32_t t = lane_perm(h, 8u * (lane & 7u));  // lane 8 (r % 8) + i reads group i
+        if ((lane >> 3) == (uint32_t)(r & 7)) res[r >> 3] = t ? t - 1u : 32u;
     }
-    return res;
 }
 
-// Pool index of x in B[g0, g1), or kNoPos, for the probe of every lane of the wave (on = false: no
-// probe), through the top level top[j - tj0] = B[kTopStride j] in LDS, the middle level
+// Pool index of x in B[g0, g1), or kNoPos, for the NP probes of every lane of the wave (on = false:
+// no probe), through the top level top[j - tj0] = B[kTopStride j] in LDS, the middle level
 // mid[m] = idx[kMidIdx m] and the index idx[k] = B[32 k] (values in the compare space), with the
 // global reads made by groups of lanes.  The windows it reads each lie in one aligned block:
 // the middle-level entries over one top-level interval are those of one 16-entry block (top
 // entries are every 16th middle entry), likewise the index entries under one middle-level
 // interval, and the candidates of B one 32-element line.
-__device__ __forceinline__ uint64_t coop_index_probe_top(bool on, const uint32_t *__restrict__ B, uint64_t pool,
-                                                         const uint32_t *__restrict__ idx,
-                                                         const uint32_t *__restrict__ mid, const uint32_t *top,
-                                                         uint64_t tj0, uint64_t g0, uint64_t g1, uint32_t x, uint32_t f)
+template <int NP>
+__device__ __forceinline__ void coop_index_probe_top(const bool (&on_)[NP], const uint32_t *__restrict__ B, uint64_t pool,
+                                                     const uint32_t *__restrict__ idx, const uint32_t *__restrict__ mid,
+                                                     const uint32_t *top, uint64_t tj0, const uint64_t (&g0)[NP],
+                                                     const uint64_t (&g1)[NP], const uint32_t (&x)[NP], uint32_t f,
+                                                     uint64_t (&q)[NP])
 {
-    on = on && g0 < g1;
-    uint64_t L = 0, H = 0, mb = 0, me = 0;
-    if (on) {
-        const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;
-        const uint64_t jb = (kb + kTopIdx - 1) / kTopIdx, je = (ke + kTopIdx - 1) / kTopIdx;
-        uint64_t lo = jb, hi = je;  // first top entry > x
-        while (lo < hi) {
-            const uint64_t m = (lo + hi) >> 1;
-            if (top[m - tj0] <= x) lo = m + 1; else hi = m;
+    bool on[NP];
+    uint64_t L[NP], H[NP], mb[NP], me[NP];
+    uint32_t mblk[NP], mlh[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        on[k] = on_[k] && g0[k] < g1[k];
+        L[k] = H[k] = mb[k] = me[k] = 0;
+        if (on[k]) {
+            const uint64_t kb = (g0[k] + 31) / 32, ke = (g1[k] + 31) / 32;
+            const uint64_t jb = (kb + kTopIdx - 1) / kTopIdx, je = (ke + kTopIdx - 1) / kTopIdx;
+            uint64_t lo = jb, hi = je;  // first top entry > x
+            while (lo < hi) {
+                const uint64_t m = (lo + hi) >> 1;
+                if (top[m - tj0] <= x[k]) lo = m + 1; else hi = m;
+            }
+            L[k] = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
+            H[k] = (lo < je) ? kTopIdx * lo : ke;
+            mb[k] = (L[k] + kMidIdx - 1) / kMidIdx;
+            me[k] = (H[k] + kMidIdx - 1) / kMidIdx;
         }
-        L = (lo > jb) ? kTopIdx * (lo - 1) + 1 : kb;
-        H = (lo < je) ? kTopIdx * lo : ke;
-        mb = (L + kMidIdx - 1) / kMidIdx;
-        me = (H + kMidIdx - 1) / kMidIdx;
+        // middle level: [mb, me) inside block mb / 16
+        mblk[k] = (uint32_t)(mb[k] >> 4);
+        mlh[k] = (on[k] && me[k] > mb[k]) ? (uint32_t)(mb[k] & 15u) | ((uint32_t)(me[k] - 16ull * mblk[k]) << 8) : 0u;
     }
-    // middle level: [mb, me) inside block mb / 16
-    const uint32_t mblk = (uint32_t)(mb >> 4);
-    const uint32_t mlh = (on && me > mb) ? (uint32_t)(mb & 15u) | ((uint32_t)(me - 16ull * mblk) << 8) : 0u;
-    const uint32_t mc = coop_count16(mid, (pool + kMidStride - 1) / kMidStride, mblk, mlh, x);
-    uint64_t L2 = 0, H2 = 0;
-    if (on) {
-        const uint64_t ms = mb + mc;
-        L2 = (ms > mb) ? kMidIdx * (ms - 1) + 1 : L;
-        H2 = (ms < me) ? kMidIdx * ms : H;
+    uint32_t mc[NP];
+    coop_count16<NP>(mid, (pool + kMidStride - 1) / kMidStride, mblk, mlh, x, mc);
+    uint64_t L2[NP];
+    uint32_t iblk[NP], ilh[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        uint64_t H2 = 0;
+        L2[k] = 0;
+        if (on[k]) {
+            const uint64_t ms = mb[k] + mc[k];
+            L2[k] = (ms > mb[k]) ? kMidIdx * (ms - 1) + 1 : L[k];
+            H2 = (ms < me[k]) ? kMidIdx * ms : H[k];
+        }
+        // index: [L2, H2) inside block L2 / 16; the first entry > x is L2 + count
+        iblk[k] = (uint32_t)(L2[k] >> 4);
+        ilh[k] = (on[k] && H2 > L2[k]) ? (uint32_t)(L2[k] & 15u) | ((uint32_t)(H2 - 16ull * iblk[k]) << 8) : 0u;
     }
-    // index: [L2, H2) inside block L2 / 16; the first entry > x is L2 + count
-    const uint32_t iblk = (uint32_t)(L2 >> 4);
-    const uint32_t ilh = (on && H2 > L2) ? (uint32_t)(L2 & 15u) | ((uint32_t)(H2 - 16ull * iblk) << 8) : 0u;
-    const uint32_t ic = coop_count16(idx, (pool + 31) / 32, iblk, ilh, x);
-    const uint64_t k = L2 + ic;
-    // B: the line [32(k-1), 32k) holds every candidate; positions inside [g0, g1) and the pool
-    const bool bon = on && k > 0;
-    const uint64_t base = bon ? 32 * (k - 1) : 0;
-    uint32_t blh = 0;
-    if (bon) {
-        const uint64_t e1 = g1 < pool ? g1 : pool;
-        const uint32_t lo = g0 > base ? (uint32_t)min(g0 - base, (uint64_t)32) : 0u;
-        const uint32_t hi = e1 > base ? (uint32_t)min(e1 - base, (uint64_t)32) : 0u;
-        blh = lo < hi ? lo | (hi << 8) : 0u;
+    uint32_t ic[NP];
+    coop_count16<NP>(idx, (pool + 31) / 32, iblk, ilh, x, ic);
+    uint32_t line[NP], blh[NP];
+    bool bon[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        // B: the line [32(kk-1), 32 kk) holds every candidate; positions inside [g0, g1) and the pool
+        const uint64_t kk = L2[k] + ic[k];
+        bon[k] = on[k] && kk > 0;
+        const uint64_t base = bon[k] ? 32 * (kk - 1) : 0;
+        line[k] = (uint32_t)(base >> 5);
+        blh[k] = 0;
+        if (bon[k]) {
+            const uint64_t e1 = g1[k] < pool ? g1[k] : pool;
+            const uint32_t lo = g0[k] > base ? (uint32_t)min(g0[k] - base, (uint64_t)32) : 0u;
+            const uint32_t hi = e1 > base ? (uint32_t)min(e1 - base, (uint64_t)32) : 0u;
+            blh[k] = lo < hi ? lo | (hi << 8) : 0u;
+        }
     }
-    const uint32_t pos = coop_find32(B, pool, (uint32_t)(base >> 5), blh, x, f);
-    return (bon && pos < 32) ? base + pos : kNoPos;
+    uint32_t pos[NP];
+    coop_find32<NP>(B, pool, line, blh, x, f, pos);
+#pragma unroll
+    for (int k = 0; k < NP; k++) q[k] = (bon[k] && pos[k] < 32) ? 32ull * line[k] + pos[k] : kNoPos;
 }
 
 template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, int SMODE>
-__global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
+__global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
     static_assert(KS >= 1 && KS <= 31, "lane masks hold KS bits");
     constexpr int U = NT * KS;
@@ -372,6 +404,8 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
         uint32_t cp = 0;
         uint64_t gp = 0;
         uint32_t swap = 0;
+        // the span of the tile's larger B sets, for the top level of the B-pool index (1b)
+        unsigned long long gmin = ~0ull, gmax = 0;
         if (tid < nseg) {
             const SegWin w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
             swap = (w.nb < w.na) ? 1u : 0u;  // probes come from the smaller side
@@ -383,6 +417,21 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             sm.seg_flags[tid] = w.flags | (swap << 2);
             sm.seg_lbase[tid] = swap ? w.a : w.b;
             sm.seg_hi[tid] = (uint32_t)(swap ? w.na : w.nb);  // larger size until narrowed
+            if (!swap && w.nb > 0) {
+                gmin = w.b;
+                gmax = w.b + w.nb;
+            }
+        }
+        if (in.b_top) {
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                gmin = min(gmin, (unsigned long long)__shfl_xor(gmin, d, kWave));
+                gmax = max(gmax, (unsigned long long)__shfl_xor(gmax, d, kWave));
+            }
+            if (lane == 0) {
+                sm.red64[wave][0] = gmin;
+                sm.red64[wave][1] = gmax;
+            }
         }
         uint32_t tot = 0;
         const uint32_t pfx = block_exclusive_scan<NT>(cp, sm.scan_scratch, &tot);
@@ -398,39 +447,12 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             sm.seg_u[nseg] = ulen;
         }
         __syncthreads();
-        if (bad) {
-            nseg = 0;
-            ulen = 0;
-            if (tid == 0 && out.error) atomicOr(out.error, 2u);
-        } else {
-            constexpr int QP = (Smem::PKEYS + 4 * NT - 1) / (4 * NT);
-            stage_windows<NT, QP>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey,
-                                  in.sflip);
-            if constexpr (BSR)
-                stage_windows<NT, QP>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst,
-                                      0u);
-        }
-        __syncthreads();
-
-        // ---- 1b. top level of the B-pool index over the tile's larger B sets ------------------------
+        // ---- 1b. top level of the B-pool index over the tile's larger B sets, loaded together with
+        // the probe windows (one round trip for both) ---------------------------------------------------
         bool top_ok = false;
         uint64_t tj0 = 0;
-        if (in.b_top) {
-            unsigned long long gmin = ~0ull, gmax = 0;
-            if (tid < nseg && !((sm.seg_flags[tid] >> 2) & 1u) && sm.seg_hi[tid] > 0) {
-                gmin = sm.seg_lbase[tid];
-                gmax = sm.seg_lbase[tid] + sm.seg_hi[tid];
-            }
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                gmin = min(gmin, (unsigned long long)__shfl_xor(gmin, d, kWave));
-                gmax = max(gmax, (unsigned long long)__shfl_xor(gmax, d, kWave));
-            }
-            if (lane == 0) {
-                sm.red64[wave][0] = gmin;
-                sm.red64[wave][1] = gmax;
-            }
-            __syncthreads();
+        uint64_t tn = 0;
+        if (in.b_top && !bad) {
             gmin = ~0ull;
             gmax = 0;
 #pragma unroll
@@ -441,14 +463,31 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
             if (gmax > gmin) {
                 // entries j with kTopStride j in [gmin, gmax)
                 tj0 = (gmin + kTopStride - 1) / kTopStride;
-                const uint64_t tn = (gmax + kTopStride - 1) / kTopStride - tj0;
-                if (tn <= (uint64_t)kTopCap) {
-                    top_ok = true;
-                    for (uint32_t i = tid; i < (uint32_t)tn; i += NT) sm.top[i] = in.b_top[tj0 + i];
-                }
+                tn = (gmax + kTopStride - 1) / kTopStride - tj0;
+                top_ok = tn <= (uint64_t)kTopCap;
             }
-            __syncthreads();
         }
+        if (bad) {
+            nseg = 0;
+            ulen = 0;
+            if (tid == 0 && out.error) atomicOr(out.error, 2u);
+        } else {
+            // the top level goes straight to LDS (4 bytes per lane, a wave's row of 64 entries per
+            // load), in flight with the windows' loads and holding no registers meanwhile
+            if (top_ok) {
+                for (uint32_t r = tid - lane; r < (uint32_t)tn; r += NT)  // r: the wave's row (uniform)
+                    if (r + lane < (uint32_t)tn)
+                        __builtin_amdgcn_global_load_lds(in.b_top + tj0 + r + lane, sm.top + r, 4, 0, 0);
+            }
+            constexpr int QP = (Smem::PKEYS + 4 * NT - 1) / (4 * NT);
+            stage_windows<NT, QP>(in.A, in.a_pool, in.B, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pkey,
+                                  in.sflip);
+            if constexpr (BSR)
+                stage_windows<NT, QP>(in.As, in.a_pool, in.Bs, in.b_pool, sm.win_c, sm.win_g, sm.win_src, nseg, sm.pst,
+                                      0u);
+            if (top_ok) __builtin_amdgcn_s_waitcnt(0);  // the direct-to-LDS loads have landed
+        }
+        __syncthreads();
 
         // ---- 2. narrow the larger sets: one wave per segment ----------------------------------------
         // A larger set in an indexed B pool longer than kNoNarrow elements is not narrowed: its
@@ -570,28 +609,41 @@ __global__ __launch_bounds__(NT) void search_tile_kernel(PairBatch in, TilePlan 
 #pragma unroll
         for (int k = 0; k < KS; k++) qrel[k] = ~0u;
         if (top_ok && in.b_idx) {
+            // one probe per lane at a time (two at once, NP = 2, measured no faster at two probes
+            // per lane and slower at four, where the registers of the second cost occupancy)
+            constexpr int NP = 1;
             uint32_t s = s0;
 #pragma unroll
-            for (int k = 0; k < KS; k++) {
-                const uint32_t u = ub + (uint32_t)k;
-                bool on = u < ulen;
-                uint32_t x = 0;
-                uint64_t g = 0, g0 = 0, g1 = 0;
-                if (on) {
-                    while (u >= sm.seg_u[s + 1]) s++;
-                    const uint32_t e = u - sm.seg_u[s];
-                    on = e < sm.seg_wa[s] && sm.seg_lb[s] == kNotStaged && !((sm.seg_flags[s] >> 2) & 1u);
-                    if (on) {
-                        x = sm.pkey[sm.seg_la[s] + e];
-                        g = sm.seg_lbase[s];
-                        g0 = g + sm.seg_lo[s];
-                        g1 = g + sm.seg_hi[s];
+            for (int k0 = 0; k0 < KS; k0 += NP) {
+                bool on[NP];
+                uint32_t x[NP];
+                uint64_t g[NP], g0[NP], g1[NP], q[NP];
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    const uint32_t u = ub + (uint32_t)(k0 + j);
+                    on[j] = k0 + j < KS && u < ulen;
+                    x[j] = 0;
+                    g[j] = g0[j] = g1[j] = 0;
+                    if (on[j]) {
+                        while (u >= sm.seg_u[s + 1]) s++;
+                        const uint32_t e = u - sm.seg_u[s];
+                        on[j] = e < sm.seg_wa[s] && sm.seg_lb[s] == kNotStaged && !((sm.seg_flags[s] >> 2) & 1u);
+                        if (on[j]) {
+                            x[j] = sm.pkey[sm.seg_la[s] + e];
+                            g[j] = sm.seg_lbase[s];
+                            g0[j] = g[j] + sm.seg_lo[s];
+                            g1[j] = g[j] + sm.seg_hi[s];
+                        }
                     }
+                    any = any || on[j];
                 }
-                if (ballot(on) != 0) {  // wave-uniform
-                    const uint64_t q = coop_index_probe_top(on, in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g0, g1,
-                                                            x, in.sflip);
-                    if (q != kNoPos) qrel[k] = (uint32_t)(q - g);
+                if (ballot(any) != 0) {  // wave-uniform
+                    coop_index_probe_top<NP>(on, in.B, in.b_pool, in.b_idx, in.b_mid, sm.top, tj0, g0, g1, x, in.sflip,
+                                             q);
+#pragma unroll
+                    for (int j = 0; j < NP; j++)
+                        if (k0 + j < KS && q[j] != kNoPos) qrel[k0 + j] = (uint32_t)(q[j] - g[j]);
                 }
             }
         }
