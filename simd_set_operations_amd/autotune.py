@@ -7,7 +7,8 @@ benchmark workloads.  Each cell is a fixed-shape CSR batch of about 2^26 input e
 2^18 pairs); every entry runs it as a device batch (plan, tile kernel, scan, placement, with
 outputs) and is timed over replayed runs with HIP events, after a check of its per-pair counts
 against the planted counts.  The table (``tuning.json``) feeds :mod:`.tuning`, which prices a
-batch per entry and picks the cheapest (``hip_auto``).
+batch per entry and picks the cheapest (``hip_auto``).  It also holds the search entries' probe
+rates against one indexed resident set (the skewed workload's shape, per smaller-set size).
 
 Run on the GPU:  python -m simd_set_operations_amd.autotune --out simd_set_operations_amd/tuning.json
 """
@@ -92,6 +93,7 @@ def run(out_path, runs=5, grid=None, algos=ALGOS, log=print):
                 log(f"[autotune] L=2^{l} skew=2^{s} sel={sel} pairs={pairs}: best {best} "
                     + " ".join(f"{a[4:]}={v:.0f}" for a, v in g.items()))
     graph = graph_split(torch, log=log)
+    resident = resident_cells(torch, runs=runs, log=log)
     table = {
         "device": torch.cuda.get_device_name(0),
         "created_unix": int(time.time()),
@@ -103,10 +105,64 @@ def run(out_path, runs=5, grid=None, algos=ALGOS, log=print):
         "algorithms": list(algos),
         "cells": cells,
         "graph_skew_split": graph,
+        "resident": resident,
     }
     with open(out_path, "w") as f:
         json.dump(table, f, indent=1)
     return table
+
+
+# resident-pool cells: one 2^24-element set over 2^28 values, indexed, searched by 2^20 probes in
+# sets of 2^4 .. 2^10 (each with a tenth planted from the large set) -- the skewed workload's shape
+RESIDENT_LOG2_SMALL = (4, 6, 8, 10)
+RESIDENT_ALGOS = ("hip_galloping", "hip_galloping@1", "hip_galloping@2", "hip_binary_search", "hip_binary_search@1",
+                  "hip_binary_search@2", "hip_simd_galloping")
+
+
+def resident_cells(torch, runs=5, log=print, log2_small=RESIDENT_LOG2_SMALL, algos=RESIDENT_ALGOS):
+    """Probes per second of the search entries against one indexed resident set, per smaller-set
+    size: GatherBatch(..., index=True) over the skewed workload's large set, counts checked against
+    the planted ones, replayed runs timed with HIP events."""
+    from . import datagen
+    from .batch import GatherBatch
+
+    stream = torch.cuda.current_stream()
+    cells = []
+    for ls in log2_small:
+        m = 1 << ls
+        P = (1 << 20) // m
+        shared = max(1, m // 10)
+        L, S = datagen.skewed(nsmall=P, small=m, shared=shared)
+        pool = S.reshape(-1)
+        a_beg = np.arange(P, dtype=np.uint64) * m
+        dL = torch.from_numpy(L.view(np.int32)).cuda()
+        g = {}
+        for algo in algos:
+            gb = GatherBatch(algo, pool, a_beg, a_beg + m, dL, np.zeros(P, np.uint64), np.full(P, L.size, np.uint64),
+                             index=True)
+            try:
+                for _ in range(2):
+                    gb.run(stream)
+                torch.cuda.synchronize()
+                _, cnt, _ = gb.results()
+                if not np.all(cnt == shared):
+                    raise RuntimeError(f"{algo}: resident-pool counts differ from the planted {shared}")
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(runs):
+                    gb.run(stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / runs
+            finally:
+                gb.close()
+            g[algo] = P * m / (ms / 1e3) / 1e9
+        best = max(g, key=g.get)
+        cells.append({"log2_small": ls, "small": m, "pairs": P, "large": int(L.size), "gprobes": g, "best": best})
+        log(f"[autotune] resident 2^24 indexed, small=2^{ls} pairs={P}: best {best} "
+            + " ".join(f"{a[4:]}={v:.1f}" for a, v in g.items()))
+        del dL
+    return cells
 
 
 GRAPH_SCALE = 20

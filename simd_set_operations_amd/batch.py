@@ -45,15 +45,17 @@ def to_device_u64(arr, torch):
     return torch.from_numpy(a.view(np.int64)).to("cuda")
 
 
-def resolve_algo(algo, a_sizes, b_sizes, bsr):
+def resolve_algo(algo, a_sizes, b_sizes, bsr, indexed=False):
     """``hip_auto``: the entry and tile shape (``name@shape``) the tuning table prices cheapest for
-    these pair sizes (its BSR twin, in the default shape, for BSR sets); any other name is
+    these pair sizes (its BSR twin, in the default shape, for BSR sets; for plain sets searched
+    through an indexed resident pool, the cheapest search entry against one); any other name is
     returned as it is."""
     if algo != "hip_auto":
         return algo
     from . import tuning
 
-    best = tuning.choose(a_sizes, b_sizes)
+    best = tuning.choose_resident(a_sizes, b_sizes) if indexed and not bsr else None
+    best = best or tuning.choose(a_sizes, b_sizes)
     if bsr:
         from .intersect import registered
 
@@ -210,10 +212,11 @@ class GatherBatch(DeviceBatch):
 
     Pools and index arrays may be numpy arrays (uploaded) or device tensors (used in place).
     ``capacity`` sizes the output (default: sum of min(|A_p|, |B_p|), computed on the host when
-    the index arrays are numpy).  ``outputs=False`` allocates no result buffer (count-only use)."""
+    the index arrays are numpy).  ``outputs=False`` allocates no result buffer (count-only use).
+    ``index=True`` indexes the B pool at creation (``index_b``)."""
 
     def __init__(self, algo, A, a_beg, a_end, B, b_beg, b_end, A_states=None, B_states=None, capacity=None,
-                 outputs=True, signed=None):
+                 outputs=True, signed=None, index=False):
         torch = _torch()
         from .intersect import algo_id
 
@@ -238,7 +241,7 @@ class GatherBatch(DeviceBatch):
         if algo == "hip_auto" and la is None:
             la = (a_end - a_beg).cpu().numpy()
             lb = (b_end - b_beg).cpu().numpy()
-        algo = self.algo = resolve_algo(algo, la, lb, A_states is not None)
+        algo = self.algo = resolve_algo(algo, la, lb, A_states is not None, indexed=index)
         self.dA = to_device_u32(A, torch)
         self.dB = to_device_u32(B, torch)
         self.a_len = int(self.dA.numel()) if len(A) else 0
@@ -262,6 +265,8 @@ class GatherBatch(DeviceBatch):
         )
         self.handle = h
         self._outputs(capacity if outputs else 0)
+        if index:
+            self.index_b()
 
     # -- k-set groups --------------------------------------------------------------------------
     def set_further(self, x_off, x_beg=None, x_end=None):
@@ -307,7 +312,5 @@ class KsetBatch(GatherBatch):
         further[first + 1] = False
         x_off = (group_off - 2 * np.arange(G + 1)).astype(np.uint64)
         super().__init__(algo, pool, beg[first], end[first], pool, beg[first + 1], end[first + 1], capacity=capacity,
-                         signed=signed)
-        if index:
-            self.index_b()
+                         signed=signed, index=index)
         self.set_further(x_off, beg[further], end[further])

@@ -7,7 +7,9 @@ predicted time of an entry is the sum over cells of the cell's elements divided 
 throughput there (averaged over the selectivities, which a caller does not know in advance).
 ``choose`` returns the cheapest entry, in the tile shape the table prices cheapest
 (``name@shape``); ``hip_auto`` batches resolve to it at creation.  The table also holds the
-graph skew-split threshold (``graph_split_threshold``).
+graph skew-split threshold (``graph_split_threshold``) and the search entries' rates against an
+indexed resident pool (``choose_resident``, for ``hip_auto`` gather batches created with
+``index=True``).
 """
 import json
 import os
@@ -82,6 +84,24 @@ def choose(a_sizes, b_sizes, table=None, algos=None):
     """The entry with the smallest predicted batch time."""
     pred = predict(a_sizes, b_sizes, table, algos)
     return min(pred, key=pred.get)
+
+
+def choose_resident(a_sizes, b_sizes, table=None):
+    """For pairs that search one indexed resident pool (GatherBatch(..., index=True)): the search
+    entry and tile shape with the smallest predicted time, from the table's resident-pool cells
+    (probes per second with a 2^24-element indexed set, per smaller-set size); each pair is priced
+    at the cell nearest its smaller size.  None without such cells."""
+    table = table or load()
+    cells = table.get("resident")
+    if not cells:
+        return None
+    sm = np.array([c["log2_small"] for c in cells], float)
+    probes = np.minimum(np.asarray(a_sizes, np.float64), np.asarray(b_sizes, np.float64))
+    idx = np.abs(np.log2(np.maximum(probes, 1.0))[:, None] - sm[None, :]).argmin(axis=1)
+    algos = sorted(set.intersection(*(set(c["gprobes"]) for c in cells)))
+    cost = {a: sum(float(probes[idx == i].sum()) / (cells[i]["gprobes"][a] * 1e9) for i in range(len(cells)))
+            for a in algos}
+    return min(cost, key=cost.get)
 
 
 def graph_split_threshold(table=None, default=32.0):

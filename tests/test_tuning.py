@@ -41,6 +41,35 @@ def test_mixed_batch_weighs_elements():
     assert tuning.choose(a, b, table=T) == min(pm, key=pm.get)
 
 
+def test_resident_choice_follows_the_smaller_set_size():
+    t = {"resident": [
+        {"log2_small": 4, "gprobes": {"a": 10.0, "b": 20.0}},
+        {"log2_small": 10, "gprobes": {"a": 30.0, "b": 5.0}},
+    ]}
+    assert tuning.choose_resident([16] * 100, [1 << 24] * 100, table=t) == "b"
+    assert tuning.choose_resident([1024] * 100, [1 << 24] * 100, table=t) == "a"
+    # a mix is priced in probes per cell: 16,000 probes at 2^4 and 102,400 at 2^10
+    mixed_a, mixed_b = [16] * 1000 + [1024] * 100, [1 << 24] * 1100
+    ca = 16000 / 10e9 + 102400 / 30e9
+    cb = 16000 / 20e9 + 102400 / 5e9
+    assert tuning.choose_resident(mixed_a, mixed_b, table=t) == ("a" if ca < cb else "b")
+    assert tuning.choose_resident([16], [1 << 24], table={"resident": []}) is None
+
+
+def test_committed_table_has_resident_cells():
+    t = tuning.load()
+    cells = t["resident"]
+    assert [c["log2_small"] for c in cells] == list(autotune.RESIDENT_LOG2_SMALL)
+    for c in cells:
+        assert set(c["gprobes"]) == set(autotune.RESIDENT_ALGOS)
+        assert all(v > 0 for v in c["gprobes"].values())
+        assert c["best"] == max(c["gprobes"], key=c["gprobes"].get)
+        assert c["pairs"] * c["small"] == 1 << 20
+    # the skewed workload's shape (4,096 x 256 against one indexed 2^24 set) resolves to a search entry
+    assert tuning.choose_resident([256] * 4096, [1 << 24] * 4096).startswith(("hip_binary_search", "hip_galloping",
+                                                                           "hip_simd_galloping"))
+
+
 def test_committed_table_covers_the_grid():
     t = tuning.load()
     grid = t["grid"]
