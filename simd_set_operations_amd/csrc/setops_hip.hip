@@ -74,6 +74,7 @@ static_assert(pad_for(kNTM * 12) >= 12 && pad_for(kNTM * 20) >= 20 && pad_for(kN
               "a lane crosses at most one pair boundary");
 constexpr int kCapB = 3072;  // search policy: larger windows staged up to this many words
 
+constexpr uint32_t kOneScanGroups = 8;      // pair-unit scans of up to this many groups run as one workgroup
 constexpr uint32_t kScanDirectGroups = 64;  // tile-count scans of up to this many groups run as one kernel
 constexpr uint64_t kMaxGrid = 1u << 22;  // grid-stride beyond this many workgroups
 constexpr int kPlaceNT = 64;   // lanes of a placement workgroup (one tile each)
@@ -435,8 +436,10 @@ uint32_t grid_for(uint64_t n, uint64_t per_block)
 int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     const uint32_t ng = (uint32_t)((h->P + kScanGroup - 1) / kScanGroup);
-    // up to kScanGroup pairs (serving steps, small gather batches): the scan is one workgroup
-    if (ng > 1) {
+    // up to kOneScanGroups groups of pairs (serving steps, small gather batches such as the skewed
+    // workload's 4,096 pairs): the scan is one workgroup, a group at a time
+    const bool one = ng <= kOneScanGroups;
+    if (!one) {
         hipLaunchKernelGGL((units_reduce_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in,
                            h->plan, 0u);
         HIP_TRY(hipGetLastError());
@@ -444,8 +447,8 @@ int launch_units_scan(setops_hip_batch *h, const TileOut &out, hipStream_t strea
                            h->plan.gsum2, (uint64_t)ng, 0, h->d_tot);
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL((units_scan_kernel<kScanGroup>), dim3(ng), dim3(kScanGroup), 0, stream, h->in, h->plan, out,
-                       ng == 1 ? 1 : 0);
+    hipLaunchKernelGGL((units_scan_kernel<kScanGroup, kOneScanGroups>), dim3(one ? 1 : ng), dim3(kScanGroup), 0, stream,
+                       h->in, h->plan, out, one ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return SETOPS_OK;
 }
@@ -559,7 +562,9 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
         int rc = launch_units_scan(h, out, stream);
         if (rc != SETOPS_OK) return rc;
     }
-    if (!h->plan.direct) {
+    // merge policy: the merge-path split of every tile start (the search policy's tile starts were
+    // recorded with the tile maps)
+    if (!h->plan.direct && h->plan.policy == POLICY_MERGE) {
         hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in,
                            h->plan);
         HIP_TRY(hipGetLastError());

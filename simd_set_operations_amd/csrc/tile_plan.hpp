@@ -127,10 +127,20 @@ __device__ __forceinline__ uint64_t pair_units(const PairBatch &in, uint32_t pol
 }
 
 // Records, for the pair owning units [u, un), the tiles it starts and ends.
+// Search policy: a tile starting inside the pair starts at probe min(t*U - u, probes) of it (the
+// split needs no search), recorded here too, so the split kernel does not run for such batches.
 __device__ __forceinline__ void map_tiles(const TilePlan &plan, uint64_t p, uint64_t u, uint64_t un)
 {
     const uint64_t U = plan.U;
-    for (uint64_t t = (u + U - 1) / U; t * U < un && t < plan.ntiles; t++) plan.tile_first[t] = (uint32_t)p;
+    const uint64_t own = un - u - plan.c0;  // the pair's probes (search policy)
+    for (uint64_t t = (u + U - 1) / U; t * U < un && t < plan.ntiles; t++) {
+        plan.tile_first[t] = (uint32_t)p;
+        if (plan.policy == POLICY_SEARCH) {
+            const uint64_t rel = t * U - u;
+            plan.tile_sa[t] = (uint32_t)(rel < own ? rel : own);
+            plan.tile_sb[t] = 0;
+        }
+    }
     for (uint64_t t = u / U; t < plan.ntiles; t++) {
         uint64_t last = (t + 1) * U;
         if (last > plan.utotal) last = plan.utotal;
@@ -280,14 +290,25 @@ __global__ __launch_bounds__(NT) void groups_scan_kernel(unsigned long long *gsu
 
 // Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.  With
 // `one_group` (a batch of at most NT pairs) the base is 0 and the group-sum kernels do not run.
-template <int NT>
+template <int NT, int MAXCHUNK>
 __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out, int one_group)
 {
+    static_assert(NT == (int)kScanGroup, "one thread per pair of a group");
     __shared__ unsigned long long wsum[NT / kWave];
-    const uint64_t p = (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
-    const unsigned long long v = (p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
-    // 64-bit block scan: wave scan, then wave totals
-    unsigned long long x = v;
+    // one_group: a single workgroup scans all pairs, thread t the C = ceil(P / NT) <= MAXCHUNK
+    // consecutive pairs from t*C (their sizes load together, one block scan of the thread sums);
+    // otherwise group blockIdx.x, one pair per thread, after the group sums' scan
+    const uint32_t C = one_group ? (uint32_t)((in.P + NT - 1) / NT) : 1u;
+    const uint64_t p0 = one_group ? (uint64_t)threadIdx.x * C : (uint64_t)blockIdx.x * kScanGroup + threadIdx.x;
+    unsigned long long vs[MAXCHUNK], local = 0;
+#pragma unroll
+    for (int c = 0; c < MAXCHUNK; c++) {
+        const uint64_t p = p0 + (uint64_t)c;
+        vs[c] = ((uint32_t)c < C && p < in.P) ? pair_units(in, plan.policy, plan.c0, p) : 0ull;
+        local += vs[c];
+    }
+    // 64-bit block scan of the thread sums: wave scan, then wave totals
+    unsigned long long x = local;
     const uint32_t l = lane_id(), w = threadIdx.x / kWave;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
@@ -296,16 +317,22 @@ __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan p
     }
     if (l == kWave - 1) wsum[w] = x;
     __syncthreads();
-    unsigned long long pre = one_group ? 0ull : plan.gsum[blockIdx.x];
-    for (int k = 0; k < (int)w; k++) pre += wsum[k];
-    if (p < in.P) {
-        const uint64_t u = pre + x - v, un = pre + x;
-        plan.uoff[p] = u;
-        if (p + 1 == in.P) plan.uoff[in.P] = un;
-        map_tiles(plan, p, u, un);
-        if (out.counts) out.counts[p] = 0;
+    unsigned long long u = one_group ? 0ull : plan.gsum[blockIdx.x];
+    for (int k = 0; k < (int)w; k++) u += wsum[k];
+    u += x - local;  // the thread's first pair starts here
+#pragma unroll
+    for (int c = 0; c < MAXCHUNK; c++) {
+        const uint64_t p = p0 + (uint64_t)c;
+        if ((uint32_t)c < C && p < in.P) {
+            const uint64_t un = u + vs[c];
+            plan.uoff[p] = u;
+            if (p + 1 == in.P) plan.uoff[in.P] = un;
+            map_tiles(plan, p, u, un);
+            if (out.counts) out.counts[p] = 0;
+            u = un;
+        }
     }
-    if (p == 0) plan_reset(out);
+    if (blockIdx.x == 0 && threadIdx.x == 0) plan_reset(out);
 }
 
 // Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
@@ -511,9 +538,32 @@ __global__ __launch_bounds__(NT) void place_tiles_kernel(PairBatch in, TilePlan 
         // results before the tile (and, for the last tile, the elements of every tile)
         const bool last = t + 1 == plan.ntiles;
         unsigned long long b = 0, el = 0;
-        for (uint32_t i = tid; i < plan.ntiles; i += NT) {
-            if (i < t) b += plan.tile_count[i];
-            if (last) el += plan.tile_elems[i];
+        // four counts per load, every load in flight together (tile arrays are padded to 256 bytes)
+        constexpr int QF = (int)kFuseScanTiles / (4 * NT);
+        static_assert(QF * 4 * NT == (int)kFuseScanTiles, "whole 16-byte loads cover the fused tile limit");
+        uint4 cv[QF], ev[QF];
+#pragma unroll
+        for (int q = 0; q < QF; q++) {
+            const uint32_t i = 4u * (tid + (uint32_t)q * NT);
+            if (i < plan.ntiles) {
+                cv[q] = *reinterpret_cast<const uint4 *>(plan.tile_count + i);
+                if (last) ev[q] = *reinterpret_cast<const uint4 *>(plan.tile_elems + i);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QF; q++) {
+            const uint32_t i = 4u * (tid + (uint32_t)q * NT);
+            if (i < plan.ntiles) {
+                const uint32_t cw[4] = {cv[q].x, cv[q].y, cv[q].z, cv[q].w};
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    if (i + j < t) b += cw[j];
+                    if (last && i + j < plan.ntiles) {
+                        const uint32_t ew[4] = {ev[q].x, ev[q].y, ev[q].z, ev[q].w};
+                        el += ew[j];
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int d = 1; d < kWave; d <<= 1) {
