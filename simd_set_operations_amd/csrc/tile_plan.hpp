@@ -289,7 +289,8 @@ __global__ __launch_bounds__(NT) void groups_scan_kernel(unsigned long long *gsu
 }
 
 // Per-group scan of pair units plus the group base: uoff, tile maps, zeroed counts.  With
-// `one_group` (a batch of at most NT pairs) the base is 0 and the group-sum kernels do not run.
+// `one_group` (a batch of at most NT * MAXCHUNK pairs) one workgroup scans the whole batch, the
+// base is 0 and the group-sum kernels do not run.
 template <int NT, int MAXCHUNK>
 __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan plan, TileOut out, int one_group)
 {
