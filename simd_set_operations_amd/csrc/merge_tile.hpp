@@ -277,14 +277,16 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             static_assert(offsetof(Smem, key) == offsetof(Smem, key_guard) + 4 * Smem::kGuardWords,
                           "key follows key_guard directly");
             const char *kb = reinterpret_cast<const char *>(sm.key_guard);
-            const uint32_t qhi = q_lo + q_n;
-            uint32_t x = q_lo;
+            // the search runs on A's probe byte offset xo = q_a4 + 4x (five VALU per step: the
+            // clamped step, B's offset, the compare and the select)
+            const uint32_t hio = q_a4 + 4u * (q_lo + q_n);
+            uint32_t xo = q_a4 + 4u * q_lo;
             for (int bit = (int)L - 1; bit >= 0; bit--) {
-                const uint32_t y = min(x + (1u << bit), qhi);
-                const uint32_t oa = q_a4 + 4u * y;
-                if (*reinterpret_cast<const uint32_t *>(kb + oa) <= *reinterpret_cast<const uint32_t *>(kb + (q_s4 - oa)))
-                    x = y;
+                const uint32_t yo = min(xo + (4u << bit), hio);
+                if (*reinterpret_cast<const uint32_t *>(kb + yo) <= *reinterpret_cast<const uint32_t *>(kb + (q_s4 - yo)))
+                    xo = yo;
             }
+            const uint32_t x = (xo - q_a4) >> 2;
             if (q_on) {
                 pa += x;  // pa / pb hold the window starts la / lb + e here
                 pb -= x;

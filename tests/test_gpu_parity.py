@@ -225,6 +225,7 @@ def test_batch_mixed_sizes(name):
     assert words == elems == int(counts.sum())
     g_off, g_cnt, _ = db.results()
     assert np.array_equal(g_cnt, counts)
+    assert np.array_equal(g_off, off)  # count-only runs still write the per-pair offsets
     db.close()
 
 
