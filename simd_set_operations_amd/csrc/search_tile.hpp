@@ -165,23 +165,63 @@ __device__ __forceinline__ uint32_t lower_bound_block_pool(const uint32_t *__res
 
 constexpr uint64_t kNoPos = ~0ull;
 
+// Entries <= x among arr[lo, hi) of a sorted array of n entries, hi - lo <= 16: the aligned 16-byte
+// chunks that cover the range load together (one round trip).
+__device__ __forceinline__ uint32_t count_le16(const uint32_t *__restrict__ arr, uint64_t n, uint64_t lo, uint64_t hi,
+                                               uint32_t x)
+{
+    uint32_t cnt = 0;
+    const uint64_t c0 = lo & ~3ull;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const uint64_t e = c0 + 4ull * (uint64_t)c;
+        if (e < hi) {
+            const uint4 v = load_chunk(arr, n, e);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t q = e + (uint64_t)j;
+                cnt += (q >= lo && q < hi && w[j] <= x) ? 1u : 0u;
+            }
+        }
+    }
+    return cnt;
+}
+
 // Pool index of x in B[g0, g1), or kNoPos, using idx[k] = B[32k] ^ f (every 32nd pool element, in
-// the compare space): a binary search over the index entries inside the range finds the only
-// 32-element line that can hold x, which is compared in full (eight 16-byte loads).  The range is
-// sorted (one set), so the index entries inside it are too.
+// the compare space) and, when given, the middle level mid[m] = idx[16 m]: a binary search over
+// the middle-level entries inside the range, then the 16 index entries under the one found (one
+// round trip), or, without the middle level, a binary search over the index entries; then the
+// only 32-element line that can hold x, compared in full (eight 16-byte loads).  The range is
+// sorted (one set), so the entries inside it are too.
 __device__ __forceinline__ uint64_t index_probe(const uint32_t *__restrict__ B, uint64_t pool,
-                                                const uint32_t *__restrict__ idx, uint64_t g0, uint64_t g1, uint32_t x,
-                                                uint32_t f)
+                                                const uint32_t *__restrict__ idx, const uint32_t *__restrict__ mid,
+                                                uint64_t g0, uint64_t g1, uint32_t x, uint32_t f)
 {
     if (g0 >= g1) return kNoPos;
     const uint64_t kb = (g0 + 31) / 32, ke = (g1 + 31) / 32;  // entries with 32k in [g0, g1)
-    uint64_t lo = kb, hi = ke;                                 // first entry > x
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (idx[mid] <= x) lo = mid + 1; else hi = mid;
+    uint64_t k;                                                // first entry > x
+    if (mid) {
+        const uint64_t mb = (kb + kMidIdx - 1) / kMidIdx, me = (ke + kMidIdx - 1) / kMidIdx;
+        uint64_t lo = mb, hi = me;  // first middle entry > x
+        while (lo < hi) {
+            const uint64_t m = (lo + hi) >> 1;
+            if (mid[m] <= x) lo = m + 1; else hi = m;
+        }
+        // the first index entry > x lies in [L, H], at most 16 entries
+        const uint64_t L = (lo > mb) ? kMidIdx * (lo - 1) + 1 : kb;
+        const uint64_t H = (lo < me) ? kMidIdx * lo : ke;
+        k = L + count_le16(idx, (pool + 31) / 32, L, H, x);
+    } else {
+        uint64_t lo = kb, hi = ke;
+        while (lo < hi) {
+            const uint64_t m = (lo + hi) >> 1;
+            if (idx[m] <= x) lo = m + 1; else hi = m;
+        }
+        k = lo;
     }
-    if (lo == 0) return kNoPos;  // kb == 0 and x < B[0] = B[g0]
-    const uint64_t base = 32 * (lo - 1);  // the line [base, base+32) holds every candidate
+    if (k == 0) return kNoPos;  // kb == 0 and x < B[0] = B[g0]
+    const uint64_t base = 32 * (k - 1);  // the line [base, base+32) holds every candidate
     uint64_t hit = kNoPos;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -678,7 +718,8 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
                     // then compare one 128-byte line of B (with the top level staged: looked up in 4a)
                     const uint64_t g = sm.seg_lbase[s];
                     const uint64_t q = top_ok ? (qrel[k] != ~0u ? g + qrel[k] : kNoPos)
-                                              : index_probe(in.B, in.b_pool, in.b_idx, g + cur, g + hi_s, x, in.sflip);
+                                              : index_probe(in.B, in.b_pool, in.b_idx, in.b_mid, g + cur, g + hi_s, x,
+                                                            in.sflip);
                     found = q != kNoPos;
                     pos = found ? (uint32_t)(q - g) : cur;
                     if constexpr (BSR)
@@ -738,7 +779,7 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
                         // a range is read only inside the pool (and its index)
                         const uint64_t g1 = min(in.x_end[j], in.b_pool), g0 = min(in.x_beg[j], g1);
                         if (in.b_idx) {
-                            found = index_probe(in.B, in.b_pool, in.b_idx, g0, g1, x, in.sflip) != kNoPos;
+                            found = index_probe(in.B, in.b_pool, in.b_idx, in.b_mid, g0, g1, x, in.sflip) != kNoPos;
                         } else if (g1 > g0) {
                             const uint32_t n = (uint32_t)(g1 - g0);
                             if constexpr (SMODE == SM_BLOCK) {
