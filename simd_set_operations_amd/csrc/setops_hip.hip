@@ -65,12 +65,17 @@ constexpr int kNumShapes = 4;
 constexpr int kMergeK[kNumShapes] = {20, 16, 24, 12};  // merge steps per lane
 constexpr int kSearchKS[kNumShapes] = {2, 1, 4, 0};    // probes per lane (0: no such shape)
 constexpr int kK = kMergeK[0];
+// BSR merge tiles: each word carries a 32-bit state beside its key, so a tile's LDS is twice a
+// plain one's; half the steps per lane keep as many tiles per CU (measured on the uneven batch in
+// BSR form: merge kernel 0.178 ms at 20 steps, 0.154-0.156 at 10, 0.159 at 12, 0.157 at 8)
+constexpr int kBsrK = 10;
 constexpr int kKS = kSearchKS[0];
 constexpr int kMaxSeg = 130;
 // every pair owns at least c0 = ceil(U / (kMaxSeg - 2)) units, so a tile touches at most kMaxSeg
 // pairs; c0 = 2K (2KS) >= K (KS): a lane crosses at most one pair boundary
 constexpr uint32_t pad_for(uint32_t U) { return (U + (kMaxSeg - 2) - 1) / (kMaxSeg - 2); }
-static_assert(pad_for(kNTM * 12) >= 12 && pad_for(kNTM * 20) >= 20 && pad_for(kNT * 1) >= 1,
+static_assert(pad_for(kNTM * 12) >= 12 && pad_for(kNTM * 20) >= 20 && pad_for(kNTM * kBsrK) >= kBsrK &&
+                  pad_for(kNT * 1) >= 1,
               "a lane crosses at most one pair boundary");
 constexpr int kCapB = 3072;  // search policy: larger windows staged up to this many words
 
@@ -281,16 +286,17 @@ int shape_count(Kind k, bool bsr)
     if (!merge_like(k)) return 3;
     return 1;
 }
-uint32_t units_per_tile(Kind k, int shape)
+uint32_t units_per_tile(Kind k, bool bsr, int shape)
 {
+    if (k == KIND_MERGE && bsr) return (uint32_t)(kNTM * kBsrK);
     return merge_like(k) ? (uint32_t)(kNTM * kMergeK[shape]) : (uint32_t)(kNT * kSearchKS[shape]);
 }
-uint32_t pad_units(Kind k, int shape) { return pad_for(units_per_tile(k, shape)); }
+uint32_t pad_units(Kind k, bool bsr, int shape) { return pad_for(units_per_tile(k, bsr, shape)); }
 // Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
 // can produce (merge: (U+1)/2 + 1; search: U).
 uint32_t stage_words(Kind k, bool bsr, int shape)
 {
-    const uint32_t U = units_per_tile(k, shape);
+    const uint32_t U = units_per_tile(k, bsr, shape);
     const uint32_t half = (merge_like(k) ? U / 2 : U) + 64;
     return bsr ? 2 * half : half;
 }
@@ -303,9 +309,9 @@ bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gath
 // and the batch has no padding tiles (the skewed workload's 4,096 pairs of 256 probes make
 // exactly 2,048 tiles of 512).  The merge tile needs its padding (a lane's steps past the end of
 // its pair run in the padding), so the merge policy always pads.
-uint32_t pad_for_batch(Kind k, int shape, uint64_t min_own)
+uint32_t pad_for_batch(Kind k, bool bsr, int shape, uint64_t min_own)
 {
-    const uint32_t pad = pad_units(k, shape);
+    const uint32_t pad = pad_units(k, bsr, shape);
     return (!merge_like(k) && min_own >= pad) ? 0u : pad;
 }
 
@@ -313,7 +319,7 @@ uint32_t pad_for_batch(Kind k, int shape, uint64_t min_own)
 // (c0: padding units per pair, counted in utotal).
 int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal, uint32_t c0)
 {
-    const uint64_t U = units_per_tile(h->kind, h->shape);
+    const uint64_t U = units_per_tile(h->kind, h->bsr, h->shape);
     const uint64_t ntiles = std::max<uint64_t>(1, (utotal + U - 1) / U);
     if (ntiles >= 0x7FFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu tiles", (unsigned long long)ntiles);
     if (P >= 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
@@ -496,12 +502,12 @@ void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream
                            h->in, h->plan, out, (uint32_t)t0);
 }
 
-// The plain merge entry in the handle's tile shape (other merge entries have the default only).
+// The merge entry in the handle's tile shape (plain sets; BSR merge tiles take kBsrK steps).
 template <bool BSR, bool CO, bool GA>
 void launch_merge_shape(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     if constexpr (BSR) {
-        launch_merge_t<kMergeK[0], BSR, CO, GA, LP_MERGE>(h, out, stream);
+        launch_merge_t<kBsrK, BSR, CO, GA, LP_MERGE>(h, out, stream);
     } else {
         switch (h->shape) {
         case 1: launch_merge_t<kMergeK[1], BSR, CO, GA, LP_MERGE>(h, out, stream); break;
@@ -763,7 +769,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->plan.single = 0;
     h->plan.direct = 0;
     int rc;
-    const uint32_t pad = pad_units(h->kind, h->shape);
+    const uint32_t pad = pad_units(h->kind, h->bsr, h->shape);
     if (!gather && merge_like(e->kind)) {
         rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)pad * npairs, pad);
     } else {
@@ -772,7 +778,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
         rc = batch_reserve(h, npairs, 1, pad);
         uint64_t utotal = 0, small = 0;
         if (rc == SETOPS_OK) rc = device_unit_total(h, &utotal, &small);
-        const uint32_t c0 = pad_for_batch(h->kind, h->shape, small ? 0 : pad);
+        const uint32_t c0 = pad_for_batch(h->kind, h->bsr, h->shape, small ? 0 : pad);
         if (rc == SETOPS_OK) rc = batch_reserve(h, npairs, utotal - (uint64_t)(pad - c0) * npairs, c0);
     }
     if (rc != SETOPS_OK) {
@@ -846,9 +852,9 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
     const uint64_t units =
-        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, h->shape);
+        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, e->bsr, h->shape);
     h->plan.single = 1;
-    int rc = batch_reserve(h, 1, units, pad_units(e->kind, h->shape));
+    int rc = batch_reserve(h, 1, units, pad_units(e->kind, e->bsr, h->shape));
     if (rc != SETOPS_OK) return rc;
     h->in.A = dA;
     h->in.As = dAs;
@@ -906,8 +912,8 @@ int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na,
     h->use_graph = false;
     h->in = PairBatch{};
     h->plan = TilePlan{};
-    const uint64_t U = units_per_tile(e->kind, h->shape);
-    const uint64_t units = work + pad_units(e->kind, h->shape);
+    const uint64_t U = units_per_tile(e->kind, e->bsr, h->shape);
+    const uint64_t units = work + pad_units(e->kind, e->bsr, h->shape);
     const uint64_t ntiles = std::max<uint64_t>(1, (units + U - 1) / U);
     const size_t ws = align_up(ws_layout(h, 1, ntiles, base), 256);
     const size_t stage_bytes = (size_t)ntiles * h->plan.stage_words * 4;
@@ -917,7 +923,7 @@ int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na,
     h->plan.stage = reinterpret_cast<uint32_t *>(base + ws);
     h->plan.utotal = units;
     h->plan.ntiles = (uint32_t)ntiles;
-    h->plan.c0 = pad_units(e->kind, h->shape);
+    h->plan.c0 = pad_units(e->kind, e->bsr, h->shape);
     h->plan.U = (uint32_t)U;
     h->plan.policy = policy_of(e->kind);
     h->plan.single = 1;
@@ -1357,7 +1363,7 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
         own += u;
         min_own = std::min(min_own, u);
     }
-    const uint32_t c0 = pad_for_batch(handle->kind, handle->shape, min_own);
+    const uint32_t c0 = pad_for_batch(handle->kind, handle->bsr, handle->shape, min_own);
     int rc = batch_reserve(handle, npairs, own + (uint64_t)c0 * npairs, c0);
     if (rc != SETOPS_OK) return rc;
     handle->P = npairs;
