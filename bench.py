@@ -293,11 +293,12 @@ def skewed_leg(torch, steps, cpu_threads, with_cpu, entries=SKEWED_ENTRIES):
 def graph_leg(torch, cpu_threads, with_cpu, scale):
     """The graph workload: R-MAT (0.57, 0.19, 0.19, 0.05) at 2^scale vertices, edge factor 16,
     symmetrized and deduplicated.  For every edge u < v, |N(u) & N(v)| summed (= 3T), count-only,
-    on the CSR in place (gather batches): plain merge, BSR merge (popcount sum), the oriented lists
-    (= T, the independent check 3 x T), and a skew split -- galloping for edges whose larger list
-    exceeds `threshold` x the smaller, merge for the rest -- over a small threshold sweep that
-    includes the threshold the autotuner persisted in tuning.json ("tuned_split"; "best_split" is
-    the fastest on this graph).  Throughput uses the streamed element count
+    on the CSR in place (gather batches): plain merge (default tile shape, and "merge_tuned" in the
+    tile shape the autotuner persisted for graphs, when that differs), BSR merge (popcount sum), the
+    oriented lists (= T, the independent check 3 x T), and a skew split -- galloping for edges whose
+    larger list exceeds `threshold` x the smaller, the tuned merge entry for the rest -- over a small
+    threshold sweep that includes the threshold the autotuner persisted in tuning.json
+    ("tuned_split"; "best_split" is the fastest on this graph).  Throughput uses the streamed element count
     sum_{u<v} (d_u + d_v) = sum_v d_v^2."""
     from simd_set_operations_amd import datagen
     from simd_set_operations_amd.graph import EdgeIntersections, csr_to_bsr, split_by_skew
@@ -337,7 +338,19 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     ms, s3, _ = timed([e])
     e.close()
     res["merge"] = {"ms": ms, "sum": s3, "gelems": E / ms / 1e6}
-    e = EdgeIntersections("hip_branchless_merge", dcol, up, rp[1:], u, v)
+    from simd_set_operations_amd import tuning
+
+    merge = tuning.graph_merge_entry()  # the tile shape the autotuner measured on a smaller R-MAT graph
+    res["tuned_merge_entry"] = merge
+    if merge != "hip_branchless_merge":
+        e = EdgeIntersections(merge, dcol, rp[:-1], rp[1:], u, v)
+        ms, sm, _ = timed([e])
+        e.close()
+        res["merge_tuned"] = {"ms": ms, "sum": sm, "gelems": E / ms / 1e6}
+        if sm != s3:
+            log(f"graph tuned merge {merge}: sum {sm} differs from {s3}")
+            sys.exit(3)
+    e = EdgeIntersections(merge, dcol, up, rp[1:], u, v)
     ms, t, _ = timed([e])
     e.close()
     res["oriented_merge"] = {"ms": ms, "triangles": t}
@@ -346,8 +359,6 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     e.close()
     res["merge_bsr"] = {"ms": ms, "sum": s3b, "gelems": E / ms / 1e6}
     best = None
-    from simd_set_operations_amd import tuning
-
     tuned = tuning.graph_split_threshold()  # measured by the autotuner on a smaller R-MAT graph
     res["tuned_threshold"] = tuned
     sweep = [(32.0, False), (128.0, False), (512.0, False), (32.0, True), (128.0, True)]
@@ -356,7 +367,7 @@ def graph_leg(torch, cpu_threads, with_cpu, scale):
     for thr, indexed in sweep:
         sk = split_by_skew(deg[u], deg[v], thr)
         a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk, index=indexed)
-        b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
+        b = EdgeIntersections(merge, dcol, rp[:-1], rp[1:], u, v, select=~sk)
         ms, sh, _ = timed([a, b])
         a.close()
         b.close()

@@ -1,5 +1,6 @@
 """Autotuner: batched throughput of every registered 2-set entry and tile shape over a grid of pair
-shapes, and the skew threshold that splits a graph's edges between the search and merge entries.
+shapes, and, for graph edge intersections, the merge tile shape and the skew threshold that splits
+the edges between the search and merge entries.
 
 Grid: larger-set size 2^6 .. 2^14, skew (larger / smaller) 2^0 .. 2^10, selectivity 0.1 / 0.5 /
 0.9, density 0.1 (element space = 10 x larger), the planted-shared-then-disjoint data of the
@@ -167,14 +168,17 @@ def resident_cells(torch, runs=5, log=print, log2_small=RESIDENT_LOG2_SMALL, alg
 
 GRAPH_SCALE = 20
 GRAPH_THRESHOLDS = (8.0, 16.0, 32.0, 64.0, 128.0, 512.0)
+GRAPH_MERGE_SHAPES = ("hip_branchless_merge", "hip_branchless_merge@1", "hip_branchless_merge@2",
+                      "hip_branchless_merge@3")
 
 
 def graph_split(torch, scale=GRAPH_SCALE, thresholds=GRAPH_THRESHOLDS, reps=3, log=print):
-    """The skew threshold for graph edge intersections: edges whose larger neighbour list is more
-    than T times the smaller go to hip_galloping over the indexed CSR, the rest to
-    hip_branchless_merge (count-only, as the triangle count runs them).  Times every T on an
-    R-MAT graph (0.57, 0.19, 0.19, 0.05) at 2^scale vertices, edge factor 16, checks that every
-    split gives the plain merge's sum, and returns the fastest T with the timings."""
+    """The merge tile shape and the skew threshold for graph edge intersections (count-only, as the
+    triangle count runs them).  On an R-MAT graph (0.57, 0.19, 0.19, 0.05) at 2^scale vertices,
+    edge factor 16: every hip_branchless_merge tile shape over all edges (the fastest is
+    "merge_entry"), then every T, edges whose larger neighbour list is more than T times the
+    smaller going to hip_galloping over the indexed CSR and the rest to the merge entry.  Every
+    run's sum is checked against the plain merge's; returns the fastest T with the timings."""
     from . import datagen
     from .graph import EdgeIntersections, split_by_skew
 
@@ -200,11 +204,23 @@ def graph_split(torch, scale=GRAPH_SCALE, thresholds=GRAPH_THRESHOLDS, reps=3, l
     plain = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v)
     ms_plain, total = timed([plain])
     plain.close()
-    res = {"scale": scale, "edges": int(u.size), "plain_merge_ms": ms_plain, "ms": {}}
+    res = {"scale": scale, "edges": int(u.size), "plain_merge_ms": ms_plain, "ms": {}, "merge_ms": {}}
+    # the merge entry's tile shape for gather batches over one CSR (their tiles stage every pair's
+    # windows separately, so the pair-shape cells, measured on CSR batches, do not price them)
+    for name in GRAPH_MERGE_SHAPES:
+        m = EdgeIntersections(name, dcol, rp[:-1], rp[1:], u, v)
+        ms, t = timed([m])
+        m.close()
+        if t != total:
+            raise RuntimeError(f"graph merge {name}: sum {t} differs from the plain merge's {total}")
+        res["merge_ms"][name] = ms
+        log(f"[autotune] graph 2^{scale} merge {name}: {ms:.3f} ms")
+    merge = min(res["merge_ms"], key=res["merge_ms"].get)
+    res["merge_entry"] = merge
     for thr in thresholds:
         sk = split_by_skew(deg[u], deg[v], thr)
         a = EdgeIntersections("hip_galloping", dcol, rp[:-1], rp[1:], u, v, select=sk, index=True)
-        b = EdgeIntersections("hip_branchless_merge", dcol, rp[:-1], rp[1:], u, v, select=~sk)
+        b = EdgeIntersections(merge, dcol, rp[:-1], rp[1:], u, v, select=~sk)
         ms, t = timed([a, b])
         a.close()
         b.close()

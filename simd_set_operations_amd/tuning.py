@@ -104,6 +104,17 @@ def choose_resident(a_sizes, b_sizes, table=None):
     return min(cost, key=cost.get)
 
 
+def graph_merge_entry(table=None, default="hip_branchless_merge"):
+    """The merge entry (with its tile shape) the autotuner measured fastest for graph edge
+    intersections (gather batches over one CSR), or `default` without a measurement."""
+    try:
+        table = table or load()
+    except FileNotFoundError:
+        return default
+    g = table.get("graph_skew_split")
+    return str(g["merge_entry"]) if g and "merge_entry" in g else default
+
+
 def graph_split_threshold(table=None, default=32.0):
     """The skew threshold the autotuner measured for graph edge intersections (larger list over
     smaller beyond which an edge goes to the search entry), or `default` without a measurement."""

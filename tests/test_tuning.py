@@ -106,6 +106,20 @@ def test_graph_split_threshold_from_the_table():
     assert tuning.graph_split_threshold(table=t) == 64.0
 
 
+def test_graph_merge_entry_from_the_table():
+    assert tuning.graph_merge_entry(table={"cells": []}) == "hip_branchless_merge"
+    t = {"cells": [], "graph_skew_split": {"threshold": 32.0, "merge_entry": "hip_branchless_merge@2",
+                                           "merge_ms": {"hip_branchless_merge": 2.0, "hip_branchless_merge@2": 1.8}}}
+    assert tuning.graph_merge_entry(table=t) == "hip_branchless_merge@2"
+
+
+def test_committed_table_graph_section():
+    g = tuning.load()["graph_skew_split"]
+    assert set(g["merge_ms"]) == set(autotune.GRAPH_MERGE_SHAPES)
+    assert g["merge_entry"] == min(g["merge_ms"], key=g["merge_ms"].get)
+    assert str(g["threshold"]) in g["ms"] and g["threshold"] == float(min(g["ms"], key=g["ms"].get))
+
+
 @pytest.mark.parametrize("larger,smaller,sel", [(64, 64, 0.5), (1024, 8, 0.9), (256, 1, 1.0), (100, 30, 0.0)])
 def test_fixed_batch_shapes_and_planted_counts(larger, smaller, sel):
     A, a_off, B, b_off, shared = datagen.fixed_batch(50, larger, smaller, sel)
