@@ -9,7 +9,8 @@
 //      emitting on an A step whose value equals the current B value (exactly once per match,
 //      independent of where tile and lane boundaries fall),
 //   4. block scan of per-lane match counts; results written in order to the tile's staging slot
-//      straight from LDS,
+//      from LDS (plain merge: first gathered in order into the staged B words, then written in
+//      whole rows),
 //   5. tile count, per-pair counts and in-tile offsets.
 //
 // The same kernel handles the BSR representation (key = base, payload = 32-bit state): a base
@@ -167,7 +168,7 @@ template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int
 __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR, GATHER, LP != LP_MERGE> &sm,
                                                    const TilePlan &plan, const TileOut &out, uint32_t t,
                                                    uint32_t nseg, uint32_t ulen, uint32_t p_first, uint32_t in_sflip,
-                                                   uint32_t nstaged)
+                                                   uint32_t nstaged, uint32_t bstart)
 {
     constexpr bool BLOCKP = LP != LP_MERGE;
     using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
@@ -393,9 +394,12 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         }
         __syncthreads();
 
+        // plain merge: results are gathered in order into the B part of the staged words (key[bstart
+        // ..)), which the output phase no longer reads, and copied out in whole rows
+        constexpr bool COMPACT = !BLOCKP && !BSR && !COUNT_ONLY;
         // int32 sets: the keys go back to their given bits before they are written out (the merges
         // that compared them are done: the scan above waited for every wave)
-        if constexpr (!COUNT_ONLY && !BSR) {
+        if constexpr (!COUNT_ONLY && !BSR && !COMPACT) {
             if (in_sflip) {
                 for (uint32_t i = tid; i < nstaged; i += NT) sm.key[i] ^= in_sflip;
                 __syncthreads();
@@ -407,7 +411,8 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
         // tile planned from sizes that disagree with the device offsets may not, and then writes
         // nothing (error bit 1) rather than past its slot.
         const uint32_t slot_keys = BSR ? plan.stage_words / 2 : plan.stage_words;
-        if (tot > slot_keys) {
+        const uint32_t lds_room = COMPACT ? (nstaged > bstart ? nstaged - bstart : 0u) : ~0u;
+        if (tot > slot_keys || tot > lds_room) {
             if (tid == 0 && out.error) atomicOr(out.error, 2u);
             tot = 0;
             smask = 0;
@@ -449,16 +454,18 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
             } else {
                 // the A element taken at step k is the lane's A element number popc(tmask below k),
                 // counted from pa0 (a lane that moved to the next pair started there and has no
-                // steps below kt); results are written at 32-bit byte offsets from the slot
-                char *dkb = reinterpret_cast<char *>(dk);
-                uint32_t off = 4u * pos;
+                // steps below kt).  Each result goes to key[bstart + its tile position] (B words:
+                // a tile has no more results than staged B elements, and A words are all that the
+                // loop reads), then the tile's results leave in order, a row of NT words per store
+                uint32_t w = bstart + pos;
                 uint32_t sm_ = smask;
                 while (sm_) {
                     const uint32_t k = __builtin_ctz(sm_);
                     sm_ &= sm_ - 1;
-                    *reinterpret_cast<uint32_t *>(dkb + off) = sm.key[pa0 + __popc(tmask & ((1u << k) - 1u))];
-                    off += 4u;
+                    sm.key[w++] = sm.key[pa0 + __popc(tmask & ((1u << k) - 1u))];
                 }
+                __syncthreads();
+                for (uint32_t i = tid; i < tot; i += NT) dk[i] = sm.key[bstart + i] ^ in_sflip;
             }
         }
 
@@ -542,6 +549,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         if (t == 0 && tid == 0 && out.error && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
         SegWin w{};
         uint32_t staged_words = 0;  // CSR: words loaded straight into LDS (flipped in place for int32 sets)
+        uint32_t bstart = 0;        // key index where the staged B words start
 
         // ---- 2. stage the windows --------------------------------------------------------------
         if constexpr (!GATHER) {
@@ -554,6 +562,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             const uint32_t cA = (shift_a + n_a + 3) / 4, cB = (shift_b + n_b + 3) / 4;
             const uint32_t boff = 4u * cA;
             staged_words = 4u * (cA + cB);
+            bstart = boff;
             // A run at key[shift_a ...], B run at key[boff + shift_b ...] (boff = the A run's chunks),
             // loaded straight into LDS; the segment table's loads are in flight meanwhile
             if (!bad) {
@@ -599,6 +608,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
             }
             if (tid == 0) sm.win_c[2 * nseg] = totA + totB;
             staged_words = 4u * (totA + totB);
+            bstart = 4u * totA;
             __syncthreads();
             constexpr int QWIN = (Smem::KEYS + 4 * NT - 1) / (4 * NT);
             if (!bad) {
@@ -634,7 +644,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         }
 
         merge_tile_compute<NT, K, MAXSEG, BSR, COUNT_ONLY, GATHER, LP>(sm, plan, out, t, nseg, ulen, p_first,
-                                                                       in.sflip, staged_words);
+                                                                       in.sflip, staged_words, bstart);
     }
 }
 
