@@ -71,11 +71,16 @@ constexpr int kK = kMergeK[0];
 constexpr int kBsrK = 10;
 constexpr int kKS = kSearchKS[0];
 constexpr int kMaxSeg = 130;
-// every pair owns at least c0 = ceil(U / (kMaxSeg - 2)) units, so a tile touches at most kMaxSeg
-// pairs; c0 = 2K (2KS) >= K (KS): a lane crosses at most one pair boundary
-constexpr uint32_t pad_for(uint32_t U) { return (U + (kMaxSeg - 2) - 1) / (kMaxSeg - 2); }
-static_assert(pad_for(kNTM * 12) >= 12 && pad_for(kNTM * 20) >= 20 && pad_for(kNTM * kBsrK) >= kBsrK &&
-                  pad_for(kNT * 1) >= 1,
+// gather merge tiles stage every pair's windows as whole 16-byte chunks, so their LDS holds 16
+// words of slack per segment: fewer segments per tile, more tiles per CU (R-MAT scale-22 edges,
+// count-only merge: 587 ms at 130 segments, 490 at 66, 431 at 50 and at 34)
+constexpr int kMaxSegGather = 50;
+// every pair owns at least c0 = ceil(U / (maxseg - 2)) units, so a tile touches at most maxseg
+// pairs; c0 >= K (KS): a lane crosses at most one pair boundary
+constexpr uint32_t pad_for(uint32_t U, int maxseg) { return (U + (maxseg - 2) - 1) / (maxseg - 2); }
+static_assert(pad_for(kNTM * 12, kMaxSeg) >= 12 && pad_for(kNTM * 20, kMaxSeg) >= 20 &&
+                  pad_for(kNTM * kBsrK, kMaxSeg) >= kBsrK && pad_for(kNT * 1, kMaxSeg) >= 1 &&
+                  pad_for(kNTM * 12, kMaxSegGather) >= 12 && pad_for(kNTM * 24, kMaxSegGather) >= 24,
               "a lane crosses at most one pair boundary");
 constexpr int kCapB = 3072;  // search policy: larger windows staged up to this many words
 
@@ -291,7 +296,12 @@ uint32_t units_per_tile(Kind k, bool bsr, int shape)
     if (k == KIND_MERGE && bsr) return (uint32_t)(kNTM * kBsrK);
     return merge_like(k) ? (uint32_t)(kNTM * kMergeK[shape]) : (uint32_t)(kNT * kSearchKS[shape]);
 }
-uint32_t pad_units(Kind k, bool bsr, int shape) { return pad_for(units_per_tile(k, bsr, shape)); }
+// Segments a tile may touch: gather merge tiles have the smaller limit.
+int max_seg(Kind k, bool gather) { return (gather && merge_like(k)) ? kMaxSegGather : kMaxSeg; }
+uint32_t pad_units(Kind k, bool bsr, bool gather, int shape)
+{
+    return pad_for(units_per_tile(k, bsr, shape), max_seg(k, gather));
+}
 // Words of one tile's staging slot: keys then (BSR) states, each >= the most results a tile
 // can produce (merge: (U+1)/2 + 1; search: U).
 uint32_t stage_words(Kind k, bool bsr, int shape)
@@ -309,9 +319,9 @@ bool closed_form(const setops_hip_batch *h) { return h->plan.single || (!h->gath
 // and the batch has no padding tiles (the skewed workload's 4,096 pairs of 256 probes make
 // exactly 2,048 tiles of 512).  The merge tile needs its padding (a lane's steps past the end of
 // its pair run in the padding), so the merge policy always pads.
-uint32_t pad_for_batch(Kind k, bool bsr, int shape, uint64_t min_own)
+uint32_t pad_for_batch(Kind k, bool bsr, bool gather, int shape, uint64_t min_own)
 {
-    const uint32_t pad = pad_units(k, bsr, shape);
+    const uint32_t pad = pad_units(k, bsr, gather, shape);
     return (!merge_like(k) && min_own >= pad) ? 0u : pad;
 }
 
@@ -488,7 +498,7 @@ template <int K, bool BSR, bool CO, bool GA, int LP>
 void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
-        hipLaunchKernelGGL((merge_tile_kernel<kNTM, K, kMaxSeg, BSR, CO, GA, LP>),
+        hipLaunchKernelGGL((merge_tile_kernel<kNTM, K, GA ? kMaxSegGather : kMaxSeg, BSR, CO, GA, LP>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNTM), 0, stream,
                            h->in, h->plan, out, (uint32_t)t0);
 }
@@ -769,7 +779,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->plan.single = 0;
     h->plan.direct = 0;
     int rc;
-    const uint32_t pad = pad_units(h->kind, h->bsr, h->shape);
+    const uint32_t pad = pad_units(h->kind, h->bsr, h->gather, h->shape);
     if (!gather && merge_like(e->kind)) {
         rc = batch_reserve(h, npairs, a_len + b_len + (uint64_t)pad * npairs, pad);
     } else {
@@ -778,7 +788,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
         rc = batch_reserve(h, npairs, 1, pad);
         uint64_t utotal = 0, small = 0;
         if (rc == SETOPS_OK) rc = device_unit_total(h, &utotal, &small);
-        const uint32_t c0 = pad_for_batch(h->kind, h->bsr, h->shape, small ? 0 : pad);
+        const uint32_t c0 = pad_for_batch(h->kind, h->bsr, h->gather, h->shape, small ? 0 : pad);
         if (rc == SETOPS_OK) rc = batch_reserve(h, npairs, utotal - (uint64_t)(pad - c0) * npairs, c0);
     }
     if (rc != SETOPS_OK) {
@@ -852,9 +862,9 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
     if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
     const uint64_t units =
-        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, e->bsr, h->shape);
+        (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, e->bsr, false, h->shape);
     h->plan.single = 1;
-    int rc = batch_reserve(h, 1, units, pad_units(e->kind, e->bsr, h->shape));
+    int rc = batch_reserve(h, 1, units, pad_units(e->kind, e->bsr, false, h->shape));
     if (rc != SETOPS_OK) return rc;
     h->in.A = dA;
     h->in.As = dAs;
@@ -913,7 +923,7 @@ int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na,
     h->in = PairBatch{};
     h->plan = TilePlan{};
     const uint64_t U = units_per_tile(e->kind, e->bsr, h->shape);
-    const uint64_t units = work + pad_units(e->kind, e->bsr, h->shape);
+    const uint64_t units = work + pad_units(e->kind, e->bsr, false, h->shape);
     const uint64_t ntiles = std::max<uint64_t>(1, (units + U - 1) / U);
     const size_t ws = align_up(ws_layout(h, 1, ntiles, base), 256);
     const size_t stage_bytes = (size_t)ntiles * h->plan.stage_words * 4;
@@ -923,7 +933,7 @@ int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na,
     h->plan.stage = reinterpret_cast<uint32_t *>(base + ws);
     h->plan.utotal = units;
     h->plan.ntiles = (uint32_t)ntiles;
-    h->plan.c0 = pad_units(e->kind, e->bsr, h->shape);
+    h->plan.c0 = pad_units(e->kind, e->bsr, false, h->shape);
     h->plan.U = (uint32_t)U;
     h->plan.policy = policy_of(e->kind);
     h->plan.single = 1;
@@ -1363,7 +1373,7 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
         own += u;
         min_own = std::min(min_own, u);
     }
-    const uint32_t c0 = pad_for_batch(handle->kind, handle->bsr, handle->shape, min_own);
+    const uint32_t c0 = pad_for_batch(handle->kind, handle->bsr, handle->gather, handle->shape, min_own);
     int rc = batch_reserve(handle, npairs, own + (uint64_t)c0 * npairs, c0);
     if (rc != SETOPS_OK) return rc;
     handle->P = npairs;
