@@ -212,6 +212,14 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
                             const uint64_t *d_b_beg, const uint64_t *d_b_end, uint64_t npairs, const uint64_t *h_na,
                             const uint64_t *h_nb);
 
+/* Grows the handle's device workspace and result staging ahead of time for up to `npairs` pairs
+ * and `units` work units (merge entries: |A| + |B| per pair, search entries: min(|A|, |B|)), so
+ * that later rebinds and runs within those sizes allocate nothing; growing during a run frees and
+ * reallocates device memory, which waits for the device.  For request loops that rebind one
+ * handle every step; the reference has no counterpart.  The previous run must be complete or on
+ * the handle's last stream. */
+int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t units);
+
 /* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
  *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
  *                          both NULL selects count-only mode (out_cap ignored)

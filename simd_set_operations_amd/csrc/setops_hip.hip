@@ -414,11 +414,10 @@ int batch_alloc(setops_hip_batch *h, uint64_t P, uint64_t ntiles)
     return SETOPS_OK;
 }
 
-// Per-tile result staging, allocated on the first run that writes results (count-only runs
-// never need it), grow-only.
-int stage_reserve(setops_hip_batch *h)
+// Per-tile result staging for nt tiles, allocated on the first run that writes results
+// (count-only runs never need it), grow-only.
+int stage_grow(setops_hip_batch *h, uint64_t nt)
 {
-    const uint64_t nt = h->plan.ntiles;
     if (h->stage && h->stage_tiles >= nt) {
         h->plan.stage = h->stage;
         return SETOPS_OK;
@@ -439,6 +438,8 @@ int stage_reserve(setops_hip_batch *h)
     h->plan.stage = h->stage;
     return SETOPS_OK;
 }
+
+int stage_reserve(setops_hip_batch *h) { return stage_grow(h, h->plan.ntiles); }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
@@ -1382,6 +1383,36 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
     handle->in.a_end = d_a_end;
     handle->in.b_beg = d_b_beg;
     handle->in.b_end = d_b_end;
+    return SETOPS_OK;
+}
+
+int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t units)
+{
+    if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
+    if (npairs >= 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "too many pairs: %llu", (unsigned long long)npairs);
+    setops_hip_batch *h = handle;
+    const uint64_t U = units_per_tile(h->kind, h->bsr, h->shape);
+    const uint64_t pad = pad_units(h->kind, h->bsr, h->gather, h->shape);
+    const uint64_t ntiles = std::max<uint64_t>(1, (units + pad * std::max<uint64_t>(npairs, 1) + U - 1) / U);
+    if (ntiles >= 0x7FFFFFFFull) return fail(SETOPS_ERR_INVALID, "too many units: %llu", (unsigned long long)units);
+    if (!h->ws || npairs > h->cap_pairs || ntiles > h->cap_tiles) {
+        // ws_layout repoints the plan's arrays; every run rewrites them, and a cached launch graph
+        // keyed on the old pointers is not replayed
+        if (h->ws) {
+            if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+            HIP_TRY(hipFree(h->ws));
+            h->ws = nullptr;
+        }
+        const uint64_t cp = std::max<uint64_t>(npairs, h->cap_pairs), ct = std::max<uint64_t>(ntiles, h->cap_tiles);
+        int rc = batch_alloc(h, cp, ct);
+        if (rc != SETOPS_OK) return rc;
+        h->cap_pairs = cp;
+        h->cap_tiles = ct;
+    }
+    const uint32_t *keep = h->plan.stage;
+    int rc = stage_grow(h, ntiles);
+    if (rc != SETOPS_OK) return rc;
+    if (h->plan.direct) h->plan.stage = const_cast<uint32_t *>(keep);
     return SETOPS_OK;
 }
 

@@ -162,6 +162,10 @@ def query_cost_words(corpus, ids):
     return need, 3 * need + 256
 
 
+RESERVE_UNITS = 1 << 24      # probes a slot's batch is sized for at start-up
+RESERVE_OUT_WORDS = 1 << 22  # result words a slot's buffers hold at start-up
+
+
 class _Slot:
     """One step in flight: a gather batch handle over the corpus pool with its own stream, output
     buffer and host staging (grow-only)."""
@@ -181,6 +185,10 @@ class _Slot:
         self._one = one
         self.d_idx = self.h_idx = None
         self.out = self.h_res = None
+        # sized ahead for a step of max_pairs queries and 2^24 probes with 2^22 result words:
+        # growing later frees and reallocates device memory, which waits for the device
+        _native.check(lib.setops_hip_batch_reserve(h, eng.max_pairs, RESERVE_UNITS))
+        self._grow(13 * eng.max_pairs + 64, RESERVE_OUT_WORDS)
         self.d_off = torch.zeros(eng.max_pairs + 1, dtype=torch.int64, device="cuda")
         self.d_cnt = torch.zeros(eng.max_pairs, dtype=torch.int32, device="cuda")
         self.d_tot = torch.zeros(2, dtype=torch.int64, device="cuda")

@@ -327,6 +327,28 @@ def test_gather_batch(name):
     gb.close()
 
 
+@pytest.mark.parametrize("name", ["hip_branchless_merge", "hip_galloping"])
+def test_batch_reserve_keeps_results(name):
+    """setops_hip_batch_reserve grows a handle's workspace and staging ahead (and shrinks
+    nothing): runs after a large and after a small reservation give the oracle's results."""
+    from simd_set_operations_amd import _native
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(93))
+    sets, pool, off, ia, ib = _gather_case(rng, 200, 1500, 2000)
+    announce(f"{name} reserve P={ia.size}")
+    gb = GatherBatch(name, pool, off[ia], off[ia + 1], pool, off[ib], off[ib + 1])
+    lib = _native.hip()
+    for npairs, units in ((8 * ia.size, 1 << 24), (1, 1), (ia.size, 0)):
+        _native.check(lib.setops_hip_batch_reserve(gb.handle, npairs, units))
+        gb.run()
+        g_off, g_cnt, g_data = gb.results()
+        for p in range(0, ia.size, 7):
+            exp = oracle.branchless_merge(sets[ia[p]], sets[ib[p]])
+            assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), (npairs, p)
+    gb.close()
+
+
 @pytest.mark.parametrize("name", BSR)
 def test_gather_batch_bsr(name):
     from simd_set_operations_amd.batch import GatherBatch
