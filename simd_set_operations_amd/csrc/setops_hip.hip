@@ -571,8 +571,14 @@ void launch_tiles_b(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipStream_t stream,
                   std::pair<hipEvent_t, hipEvent_t> *ev)
 {
-    // plan: unit prefix and tile maps, then tile starts
-    if (closed_form(h)) {
+    // plan: unit prefix and tile maps, then tile starts (one pair over several merge tiles: both
+    // in one launch)
+    const bool single_split = h->plan.single && !h->plan.direct && h->plan.policy == POLICY_MERGE;
+    if (single_split) {
+        hipLaunchKernelGGL(plan_split_single_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream,
+                           h->in, h->plan, out);
+        HIP_TRY(hipGetLastError());
+    } else if (closed_form(h)) {
         hipLaunchKernelGGL(plan_csr_kernel, dim3(grid_for(h->P, 256)), dim3(256), 0, stream, h->in, h->plan, out);
         HIP_TRY(hipGetLastError());
     } else {
@@ -581,7 +587,7 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     }
     // merge policy: the merge-path split of every tile start (the search policy's tile starts were
     // recorded with the tile maps)
-    if (!h->plan.direct && h->plan.policy == POLICY_MERGE) {
+    if (!single_split && !h->plan.direct && h->plan.policy == POLICY_MERGE) {
         hipLaunchKernelGGL(plan_split_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream, h->in,
                            h->plan);
         HIP_TRY(hipGetLastError());

@@ -336,6 +336,44 @@ __global__ __launch_bounds__(NT) void units_scan_kernel(PairBatch in, TilePlan p
     if (blockIdx.x == 0 && threadIdx.x == 0) plan_reset(out);
 }
 
+// One pair over several tiles (single mode, merge policy): the plan and the split in one launch.
+// Pair 0 owns units [0, na + nb + c0) -- every tile lies in it -- so thread t writes tile t's maps
+// and finds its merge-path split of unit t*U directly (the search of plan_split_kernel).
+__global__ void plan_split_single_kernel(PairBatch in, TilePlan plan, TileOut out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t na = plan.single_na, nb = plan.single_nb;
+    if (t == 0) {
+        plan.single_off[0] = 0;
+        plan.single_off[1] = na;
+        plan.single_off[2] = 0;
+        plan.single_off[3] = nb;
+        plan.uoff[0] = 0;
+        plan.uoff[1] = na + nb + plan.c0;
+        plan.tile_ga[plan.ntiles] = na;
+        plan.tile_gb[plan.ntiles] = nb;
+        if (out.counts) out.counts[0] = 0;
+        plan_reset(out);
+    }
+    if (t >= plan.ntiles) return;
+    plan.tile_first[t] = 0;
+    plan.tile_last[t] = 0;
+    const uint64_t rel = t * plan.U;
+    const uint64_t d = rel < na + nb ? rel : na + nb;
+    uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    const uint32_t *__restrict__ A = in.A;
+    const uint32_t *__restrict__ B = in.B;
+    const uint32_t f = in.sflip;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((A[mid] ^ f) <= (B[d - 1 - mid] ^ f)) lo = mid + 1; else hi = mid;
+    }
+    plan.tile_sa[t] = (uint32_t)lo;
+    plan.tile_sb[t] = (uint32_t)(d - lo);
+    plan.tile_ga[t] = lo;
+    plan.tile_gb[t] = d - lo;
+}
+
 // Where each tile starts inside its first pair: the merge-path split of unit t*U (merge policy:
 // A index i and B index d - i with every A[x < i] <= B[d-1-...]), or the probe index (search).
 // One thread per tile (a cooperative search costs more loads in total than it saves latency).
