@@ -73,8 +73,9 @@ constexpr int kKS = kSearchKS[0];
 constexpr int kMaxSeg = 130;
 // gather merge tiles stage every pair's windows as whole 16-byte chunks, so their LDS holds 16
 // words of slack per segment: fewer segments per tile, more tiles per CU (R-MAT scale-22 edges,
-// count-only merge: 587 ms at 130 segments, 490 at 66, 431 at 50 and at 34)
-constexpr int kMaxSegGather = 50;
+// count-only merge: 587 ms at 130 segments, 490 at 66, 431 at 50 and at 34; the BSR merge, whose
+// tiles hold states too: 600 ms at 50, 526 at 34)
+constexpr int kMaxSegGather = 34;
 // every pair owns at least c0 = ceil(U / (maxseg - 2)) units, so a tile touches at most maxseg
 // pairs; c0 >= K (KS): a lane crosses at most one pair boundary
 constexpr uint32_t pad_for(uint32_t U, int maxseg) { return (U + (maxseg - 2) - 1) / (maxseg - 2); }
