@@ -251,33 +251,48 @@ class Engine:
         for sl in self.slots:
             sl.close(self.lib)
 
-    def _launch(self, sl, qs, ids, ks, need):
-        """Enqueue one step for queries `qs` on slot `sl` (nothing waits here)."""
+    @staticmethod
+    def _stream_rows(c, ids, ks, need):
+        """Per-query step rows of a whole stream, laid out once before it is served: the pair's
+        bounds (A begin, A end, B begin, B end), the sizes, and the further sets' bounds as one
+        flat array with an offset per query (queries are admitted in arrival order, so a step's
+        queries are a contiguous range and take slices of these)."""
+        l0 = ids[:, 0]
+        l1 = ids[:, 1]
+        b0, e0 = c.beg[l0], c.beg[l0] + c.lens[l0]
+        b1, e1 = c.beg[l1], c.beg[l1] + c.lens[l1]
+        pairs = np.stack([b0, e0, b1, e1], axis=0).astype(np.uint64)  # (4, nq)
+        nx = (ks - 2).astype(np.int64)
+        xoff = np.zeros(ids.shape[0] + 1, np.int64)
+        np.cumsum(nx, out=xoff[1:])
+        X = int(xoff[-1])
+        xq = np.repeat(np.arange(ids.shape[0]), nx)
+        xj = np.arange(X) - np.repeat(xoff[:-1], nx) + 2
+        xl = ids[xq, xj] if X else np.zeros(0, np.int64)
+        xb = c.beg[xl].astype(np.uint64)
+        xe = (c.beg[xl] + c.lens[xl]).astype(np.uint64)
+        need_pfx = np.zeros(ids.shape[0] + 1, np.int64)
+        np.cumsum(need, out=need_pfx[1:])
+        return {"pairs": pairs, "na": np.ascontiguousarray(c.lens[l0], np.uint64),
+                "nb": np.ascontiguousarray(c.lens[l1], np.uint64), "xoff": xoff, "xb": xb, "xe": xe,
+                "need_pfx": need_pfx}
+
+    def _launch(self, sl, q0, q1, rows):
+        """Enqueue one step for queries q0 .. q1-1 on slot `sl` (nothing waits here)."""
         torch = self.torch
-        c = self.c
-        P = len(qs)
-        qs = np.asarray(qs, np.int64)
-        k = ks[qs]
-        l0, l1 = ids[qs, 0], ids[qs, 1]
-        nx = k - 2
-        X = int(nx.sum())
-        xq = np.repeat(np.arange(P), nx)
-        xj = np.arange(X) - np.repeat(np.cumsum(nx) - nx, nx) + 2
-        xl = ids[qs[xq], xj] if X else np.zeros(0, np.int64)
-        out_words = int(need[qs].sum()) + 64
+        P = q1 - q0
+        x0, x1 = int(rows["xoff"][q0]), int(rows["xoff"][q1])
+        X = x1 - x0
+        out_words = int(rows["need_pfx"][q1] - rows["need_pfx"][q0]) + 64
         words = 5 * P + 1 + 2 * X
         sl._grow(words, out_words)
         h = sl.h_idx.numpy().view(np.uint64)
-        h[0:P] = c.beg[l0]
-        h[P:2 * P] = c.beg[l0] + c.lens[l0]
-        h[2 * P:3 * P] = c.beg[l1]
-        h[3 * P:4 * P] = c.beg[l1] + c.lens[l1]
-        h[4 * P] = 0
-        np.cumsum(nx, out=h[4 * P + 1:5 * P + 1])
-        h[5 * P + 1:5 * P + 1 + X] = c.beg[xl]
-        h[5 * P + 1 + X:5 * P + 1 + 2 * X] = c.beg[xl] + c.lens[xl]
-        na = np.ascontiguousarray(c.lens[l0], np.uint64)
-        nb = np.ascontiguousarray(c.lens[l1], np.uint64)
+        h[0:4 * P].reshape(4, P)[:] = rows["pairs"][:, q0:q1]
+        np.subtract(rows["xoff"][q0:q1 + 1], x0, out=h[4 * P:5 * P + 1], casting="unsafe")
+        h[5 * P + 1:5 * P + 1 + X] = rows["xb"][x0:x1]
+        h[5 * P + 1 + X:5 * P + 1 + 2 * X] = rows["xe"][x0:x1]
+        na = rows["na"][q0:q1]
+        nb = rows["nb"][q0:q1]
         lib = self.lib
         with torch.cuda.stream(sl.stream):
             sl.d_idx[:words].copy_(sl.h_idx[:words], non_blocking=True)
@@ -293,7 +308,7 @@ class Engine:
             sl.h_off[:P + 1].copy_(sl.d_off[:P + 1], non_blocking=True)
             sl.h_res[:out_words].copy_(sl.out[:out_words], non_blocking=True)
             sl.done.record(sl.stream)
-        sl.queries = qs
+        sl.queries = np.arange(q0, q1)
 
     def serve(self, ids, ks, arrivals, keep_results=False, timeout_s=60.0):
         """Serve the query stream; returns (latency seconds per query, results list or None, stats)."""
@@ -307,6 +322,7 @@ class Engine:
         gc_was = gc.isenabled()
         gc.collect()
         gc.disable()  # no collector pauses inside the timed loop
+        rows = self._stream_rows(self.c, ids, ks, need)
         done_t = np.full(nq, np.nan)
         results = [None] * nq if keep_results else None
         pending = deque()
@@ -325,7 +341,9 @@ class Engine:
             if pending:  # this slot is free: its last step was finished below
                 add = adm.admit(pending, cost)
                 if add:
-                    self._launch(sl, add, ids, ks, need)
+                    if add[-1] - add[0] + 1 != len(add):
+                        raise RuntimeError("admission must take queries in arrival order")
+                    self._launch(sl, add[0], add[-1] + 1, rows)
                     steps += 1
                     pairs += len(add)
             if other.queries is not None:

@@ -50,3 +50,41 @@ def test_query_cost_follows_the_smallest_list():
     need, cost = query_cost_words(FakeCorpus, ids)
     assert need.tolist() == [7, 50]
     assert cost.tolist() == [3 * 7 + 256, 3 * 50 + 256]
+
+
+def test_step_rows_follow_the_batch_layout():
+    """A step of queries q0 .. q1-1 packs, from the stream's rows, exactly what
+    setops_hip_batch_rebind / _set_further read: A begin/end and B begin/end of every pair (the
+    query's first two lists), the further-set offsets per pair and the further sets' bounds."""
+    from types import SimpleNamespace
+
+    from simd_set_operations_amd.serving import Engine
+
+    rng = np.random.Generator(np.random.PCG64(5))
+    lens = rng.integers(1, 1000, size=50).astype(np.uint64)
+    beg = np.zeros(50, np.uint64)
+    beg[1:] = np.cumsum(lens[:-1])
+    c = SimpleNamespace(beg=beg, lens=lens)
+    nq, kmax = 40, 6
+    ks = rng.integers(2, kmax + 1, size=nq)
+    ids = np.full((nq, kmax), -1, np.int64)
+    for q in range(nq):
+        ids[q, :ks[q]] = rng.choice(50, size=ks[q], replace=False)
+    need = lens[ids[:, 0]].astype(np.int64)
+    rows = Engine._stream_rows(c, ids, ks, need)
+    for q0, q1 in ((0, 1), (3, 17), (17, 40)):
+        P = q1 - q0
+        qs = np.arange(q0, q1)
+        np.testing.assert_array_equal(rows["pairs"][:, q0:q1],
+                                      [beg[ids[qs, 0]], beg[ids[qs, 0]] + lens[ids[qs, 0]],
+                                       beg[ids[qs, 1]], beg[ids[qs, 1]] + lens[ids[qs, 1]]])
+        x0, x1 = rows["xoff"][q0], rows["xoff"][q1]
+        xs = [int(i) for q in qs for i in ids[q, 2:ks[q]]]
+        assert x1 - x0 == len(xs)
+        np.testing.assert_array_equal(rows["xb"][x0:x1], beg[xs])
+        np.testing.assert_array_equal(rows["xe"][x0:x1], beg[xs] + lens[xs])
+        np.testing.assert_array_equal(rows["xoff"][q0:q1 + 1] - x0, np.concatenate([[0], np.cumsum(ks[qs] - 2)]))
+        assert rows["need_pfx"][q1] - rows["need_pfx"][q0] == need[qs].sum()
+        np.testing.assert_array_equal(rows["na"][q0:q1], lens[ids[qs, 0]])
+        np.testing.assert_array_equal(rows["nb"][q0:q1], lens[ids[qs, 1]])
+        assert P == qs.size
