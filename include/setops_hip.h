@@ -220,6 +220,19 @@ int setops_hip_batch_rebind(setops_hip_batch *handle, const uint64_t *d_a_beg, c
  * the handle's last stream. */
 int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t units);
 
+/* One request-loop step on a gather handle, enqueued on `stream` with one call (nothing waits):
+ * copies the packed step arrays h_idx -> d_idx (uint64: a_beg[P], a_end[P], b_beg[P], b_end[P],
+ * then, with nfurther >= 0, x_off[P+1], x_beg[nfurther], x_end[nfurther]; h_idx in pinned host
+ * memory), rebinds the handle to them (h_na / h_nb: the pairs' sizes, as for
+ * setops_hip_batch_rebind), attaches the further sets (nfurther < 0: none), runs it with outputs
+ * (as setops_hip_batch_run), then enqueues the copies of d_out_off[0 .. P] to h_out_off and of
+ * d_out[0 .. copy_words) to h_out.  For serving loops that answer new queries every step; the
+ * reference has no counterpart. */
+int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint64_t *d_idx, uint64_t npairs,
+                          int64_t nfurther, const uint64_t *h_na, const uint64_t *h_nb, uint32_t *d_out,
+                          uint64_t out_cap, uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals,
+                          uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream);
+
 /* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
  *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
  *                          both NULL selects count-only mode (out_cap ignored)

@@ -92,6 +92,8 @@ HIP_SIGNATURES = {
     "setops_hip_batch_set_further": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "setops_hip_batch_rebind": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_uint64, _vp, _vp]),
     "setops_hip_batch_reserve": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint64]),
+    "setops_hip_batch_step": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _c.c_int64, _vp, _vp, _vp, _c.c_uint64, _vp,
+                                         _vp, _vp, _vp, _vp, _c.c_uint64, _vp]),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),
     "setops_hip_batch_set_timing": (_c.c_int, [_vp, _c.c_int]),

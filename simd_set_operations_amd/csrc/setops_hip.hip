@@ -1423,6 +1423,36 @@ int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t
     return SETOPS_OK;
 }
 
+int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint64_t *d_idx, uint64_t npairs,
+                          int64_t nfurther, const uint64_t *h_na, const uint64_t *h_nb, uint32_t *d_out,
+                          uint64_t out_cap, uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals,
+                          uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream)
+{
+    if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
+    if (!h_idx || !d_idx || !d_out_off || !h_out_off || (copy_words && (!h_out || !d_out)))
+        return fail(SETOPS_ERR_INVALID, "NULL argument");
+    if (copy_words > out_cap) return fail(SETOPS_ERR_INVALID, "copy_words exceeds out_cap");
+    if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t P = npairs;
+    const uint64_t words = 4 * P + (nfurther >= 0 ? P + 1 + 2 * (uint64_t)nfurther : 0);
+    HIP_TRY(hipMemcpyAsync(d_idx, h_idx, words * 8, hipMemcpyHostToDevice, s));
+    int rc = setops_hip_batch_rebind(handle, d_idx, d_idx + P, d_idx + 2 * P, d_idx + 3 * P, P, h_na, h_nb);
+    if (rc != SETOPS_OK) return rc;
+    if (nfurther >= 0) {
+        const uint64_t *xo = d_idx + 4 * P;
+        rc = setops_hip_batch_set_further(handle, xo, xo + P + 1, xo + P + 1 + nfurther);
+    } else {
+        rc = setops_hip_batch_set_further(handle, nullptr, nullptr, nullptr);
+    }
+    if (rc != SETOPS_OK) return rc;
+    rc = batch_launch(handle, d_out, nullptr, out_cap, d_out_off, d_counts, d_totals, s);
+    if (rc != SETOPS_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out_off, d_out_off, (P + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (copy_words) HIP_TRY(hipMemcpyAsync(h_out, d_out, copy_words * 4, hipMemcpyDeviceToHost, s));
+    return SETOPS_OK;
+}
+
 int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_out_states, uint64_t out_cap,
                          uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals, void *stream)
 {

@@ -279,7 +279,6 @@ class Engine:
 
     def _launch(self, sl, q0, q1, rows):
         """Enqueue one step for queries q0 .. q1-1 on slot `sl` (nothing waits here)."""
-        torch = self.torch
         P = q1 - q0
         x0, x1 = int(rows["xoff"][q0]), int(rows["xoff"][q1])
         X = x1 - x0
@@ -293,21 +292,12 @@ class Engine:
         h[5 * P + 1 + X:5 * P + 1 + 2 * X] = rows["xe"][x0:x1]
         na = rows["na"][q0:q1]
         nb = rows["nb"][q0:q1]
-        lib = self.lib
-        with torch.cuda.stream(sl.stream):
-            sl.d_idx[:words].copy_(sl.h_idx[:words], non_blocking=True)
-            base = sl.d_idx.data_ptr()
-            _native.check(lib.setops_hip_batch_rebind(
-                sl.handle, base, base + 8 * P, base + 16 * P, base + 24 * P, P, na.ctypes.data_as(_c.c_void_p),
-                nb.ctypes.data_as(_c.c_void_p)))
-            xb = base + 8 * (5 * P + 1)
-            _native.check(lib.setops_hip_batch_set_further(sl.handle, base + 32 * P, xb, xb + 8 * X))
-            _native.check(lib.setops_hip_batch_run(
-                sl.handle, sl.out.data_ptr(), None, sl.out.numel(), sl.d_off.data_ptr(), sl.d_cnt.data_ptr(),
-                sl.d_tot.data_ptr(), sl.stream.cuda_stream))
-            sl.h_off[:P + 1].copy_(sl.d_off[:P + 1], non_blocking=True)
-            sl.h_res[:out_words].copy_(sl.out[:out_words], non_blocking=True)
-            sl.done.record(sl.stream)
+        # upload, rebind, further sets, run and the copies back: one call on the slot's stream
+        _native.check(self.lib.setops_hip_batch_step(
+            sl.handle, sl.h_idx.data_ptr(), sl.d_idx.data_ptr(), P, X, na.ctypes.data, nb.ctypes.data,
+            sl.out.data_ptr(), sl.out.numel(), sl.d_off.data_ptr(), sl.d_cnt.data_ptr(), sl.d_tot.data_ptr(),
+            sl.h_off.data_ptr(), sl.h_res.data_ptr(), out_words, sl.stream.cuda_stream))
+        sl.done.record(sl.stream)
         sl.queries = np.arange(q0, q1)
 
     def serve(self, ids, ks, arrivals, keep_results=False, timeout_s=60.0):

@@ -108,6 +108,8 @@ def test_argument_errors_before_device_use():
     assert lib.setops_hip_batch_run(None, None, None, 0, None, None, None, None) == _native.SETOPS_ERR_INVALID
     assert lib.setops_hip_batch_check(None) == _native.SETOPS_ERR_INVALID
     assert lib.setops_hip_batch_reserve(None, 1, 1) == _native.SETOPS_ERR_INVALID
+    assert lib.setops_hip_batch_step(None, None, None, 1, -1, None, None, None, 0, None, None, None, None, None, 0,
+                                     None) == _native.SETOPS_ERR_INVALID
     # the int32 mode is for plain entries: the reference builds BSR from u32 values
     w, el = ctypes.c_size_t(), ctypes.c_uint64()
     rc = lib.setops_hip_intersect2_bsr(bsr | _native.SIGNED_I32, p, p, 4, p, p, 4, None, None, 0, ctypes.byref(w),
