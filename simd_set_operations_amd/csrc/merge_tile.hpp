@@ -533,20 +533,29 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         // ---- 1. descriptor and segment windows -------------------------------------------------
         // CSR: the tile's A and B runs load with its pair range (one round trip before the
         // staging is issued, not two)
+        // direct (one pair in one tile, CSR form): the tile is the whole pair and nothing planned
+        // it, so its runs, its one segment and the error word come from the plan's fields here
+        const bool direct = !GATHER && plan.direct;
         uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
         if constexpr (!GATHER) {
-            ga0 = plan.tile_ga[t];
-            ga1 = plan.tile_ga[t + 1];
-            gb0 = plan.tile_gb[t];
-            gb1 = plan.tile_gb[t + 1];
+            if (direct) {
+                ga1 = plan.single_na;
+                gb1 = plan.single_nb;
+                if (tid == 0 && out.error) *out.error = 0;  // before thread 0's own flags below
+            } else {
+                ga0 = plan.tile_ga[t];
+                ga1 = plan.tile_ga[t + 1];
+                gb0 = plan.tile_gb[t];
+                gb1 = plan.tile_gb[t + 1];
+            }
         }
-        const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
+        const uint32_t p_first = direct ? 0u : plan.tile_first[t], p_last = direct ? 0u : plan.tile_last[t];
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
         uint32_t nseg = p_last - p_first + 1;
         bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
-        if (t == 0 && tid == 0 && out.error && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
+        if (t == 0 && tid == 0 && out.error && !direct && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
         SegWin w{};
         uint32_t staged_words = 0;  // CSR: words loaded straight into LDS (flipped in place for int32 sets)
         uint32_t bstart = 0;        // key index where the staged B words start
@@ -573,7 +582,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                     stage_run_lds<NT>(in.Bs, in.b_pool, gb0 & ~3ull, cB, sm.st + boff);
                 }
             }
-            if (tid < nseg) w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            if (tid < nseg) w = direct ? direct_window(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
             if (tid < nseg) {
                 sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
                 sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);

@@ -573,9 +573,12 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
                   std::pair<hipEvent_t, hipEvent_t> *ev)
 {
     // plan: unit prefix and tile maps, then tile starts (one pair over several merge tiles: both
-    // in one launch)
+    // in one launch; one pair in one merge tile: none, the tile kernel takes the pair as it is)
     const bool single_split = h->plan.single && !h->plan.direct && h->plan.policy == POLICY_MERGE;
-    if (single_split) {
+    const bool direct_merge = h->plan.direct && h->plan.policy == POLICY_MERGE;
+    if (direct_merge) {
+        // no plan launch
+    } else if (single_split) {
         hipLaunchKernelGGL(plan_split_single_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream,
                            h->in, h->plan, out);
         HIP_TRY(hipGetLastError());

@@ -755,6 +755,25 @@ __device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan
     return seg_window(in, plan, t, p_first, nseg, s, ulen, tile_edges(plan, t));
 }
 
+// The one segment of a direct plan (one pair in one tile): the whole pair, starting and ending
+// in the tile at unit 0.
+__device__ __forceinline__ SegWin direct_window(const TilePlan &plan)
+{
+    SegWin w;
+    w.p = 0;
+    w.a = 0;
+    w.b = 0;
+    w.na = plan.single_na;
+    w.nb = plan.single_nb;
+    w.i0 = 0;
+    w.i1 = plan.single_na;
+    w.j0 = 0;
+    w.j1 = plan.single_nb;
+    w.u = 0;
+    w.flags = 3u;
+    return w;
+}
+
 // One 16-byte chunk of a pool at element index e (16-byte aligned), guarding the pool end.
 __device__ __forceinline__ uint4 load_chunk(const uint32_t *__restrict__ src, uint64_t pool, uint64_t e)
 {
