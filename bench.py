@@ -28,12 +28,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # the kernel whose time is the roofline's (per headline entry)
 DOMINANT_KERNEL = {
-    "hip_branchless_merge": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 0>",
-    "hip_shuffling": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 1>",
-    "hip_qfilter": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 2>",
-    "hip_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 1>",
-    "hip_binary_search": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 0>",
-    "hip_simd_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 2>",
+    "hip_branchless_merge": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 0, false>",
+    "hip_shuffling": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 1, false>",
+    "hip_qfilter": "setops::merge_tile_kernel<256, 20, 130, false, false, false, 2, false>",
+    "hip_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 1, false>",
+    "hip_binary_search": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 0, false>",
+    "hip_simd_galloping": "setops::search_tile_kernel<256, 2, 130, 3072, false, false, 2, false>",
 }
 
 

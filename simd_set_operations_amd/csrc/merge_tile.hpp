@@ -516,10 +516,11 @@ __device__ __forceinline__ void merge_tile_compute(MergeSmem<NT, K, MAXSEG, BSR,
 // workgroups.  Each tile's results go to its own staging slot (or nowhere in count-only mode);
 // its counts go to tile_count / tile_elems; per-pair counts and in-tile offsets are recorded for
 // the placement kernel.
-template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP = LP_MERGE>
+template <int NT, int K, int MAXSEG, bool BSR, bool COUNT_ONLY, bool GATHER, int LP = LP_MERGE, bool DIRECT = false>
 __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
     static_assert(K >= 1 && K <= 32, "lane masks hold K bits");
+    static_assert(!DIRECT || (!GATHER && !BSR), "a direct tile is one plain CSR pair");
     constexpr int U = NT * K;
     constexpr bool BLOCKP = LP != LP_MERGE;
     using Smem = MergeSmem<NT, K, MAXSEG, BSR, GATHER, BLOCKP>;
@@ -533,12 +534,13 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
         // ---- 1. descriptor and segment windows -------------------------------------------------
         // CSR: the tile's A and B runs load with its pair range (one round trip before the
         // staging is issued, not two)
-        // direct (one pair in one tile, CSR form): the tile is the whole pair and nothing planned
-        // it, so its runs, its one segment and the error word come from the plan's fields here
-        const bool direct = !GATHER && plan.direct;
+        // DIRECT (one pair in one tile, CSR form, the launch for plan.direct): the tile is the whole
+        // pair and nothing planned it, so its runs, its one segment and the error word come from the
+        // plan's fields here.  A template flag: a runtime branch here costs the batch tiles 3-5%.
+        constexpr bool direct = DIRECT;
         uint64_t ga0 = 0, ga1 = 0, gb0 = 0, gb1 = 0;
         if constexpr (!GATHER) {
-            if (direct) {
+            if constexpr (direct) {
                 ga1 = plan.single_na;
                 gb1 = plan.single_nb;
                 if (tid == 0 && out.error) *out.error = 0;  // before thread 0's own flags below
@@ -582,7 +584,7 @@ __global__ __launch_bounds__(NT) void merge_tile_kernel(PairBatch in, TilePlan p
                     stage_run_lds<NT>(in.Bs, in.b_pool, gb0 & ~3ull, cB, sm.st + boff);
                 }
             }
-            if (tid < nseg) w = direct ? direct_window(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            if (tid < nseg) w = direct ? direct_window<false>(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
             if (tid < nseg) {
                 sm.seg_la[tid] = shift_a + (uint32_t)(w.a + w.i0 - ga0);
                 sm.seg_lb[tid] = boff + shift_b + (uint32_t)(w.b + w.j0 - gb0);

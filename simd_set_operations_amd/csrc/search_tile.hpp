@@ -417,7 +417,7 @@ __device__ __forceinline__ void coop_index_probe_top(const bool (&on_)[NP], cons
     for (int k = 0; k < NP; k++) q[k] = (bon[k] && pos[k] < 32) ? 32ull * line[k] + pos[k] : kNoPos;
 }
 
-template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, int SMODE>
+template <int NT, int KS, int MAXSEG, int CAPB, bool BSR, bool COUNT_ONLY, int SMODE, bool DIRECT = false>
 __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePlan plan, TileOut out, uint32_t tile0)
 {
     static_assert(KS >= 1 && KS <= 31, "lane masks hold KS bits");
@@ -434,20 +434,25 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
         const uint32_t t = tile0 + blockIdx.x;
         if (t >= plan.ntiles) return;
         // ---- 1. segment table, probe windows -------------------------------------------------------
-        const uint32_t p_first = plan.tile_first[t], p_last = plan.tile_last[t];
+        // DIRECT (one pair in one tile, the launch for plan.direct): nothing planned the tile; the
+        // pair is its one segment
+        static_assert(!DIRECT || !BSR, "a direct tile is one plain pair");
+        constexpr bool direct = DIRECT;
+        if (direct && tid == 0 && out.error) *out.error = 0;  // before thread 0's own flags below
+        const uint32_t p_first = direct ? 0u : plan.tile_first[t], p_last = direct ? 0u : plan.tile_last[t];
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
         uint32_t nseg = p_last - p_first + 1;
         bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
-        if (t == 0 && tid == 0 && out.error && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
+        if (t == 0 && tid == 0 && out.error && !direct && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
         uint32_t cp = 0;
         uint64_t gp = 0;
         uint32_t swap = 0;
         // the span of the tile's larger B sets, for the top level of the B-pool index (1b)
         unsigned long long gmin = ~0ull, gmax = 0;
         if (tid < nseg) {
-            const SegWin w = seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            const SegWin w = direct ? direct_window<true>(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
             swap = (w.nb < w.na) ? 1u : 0u;  // probes come from the smaller side
             gp = (swap ? w.b : w.a) + w.i0;
             const uint32_t n = (uint32_t)(w.i1 - w.i0);

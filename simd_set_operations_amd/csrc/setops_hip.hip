@@ -495,10 +495,19 @@ int device_unit_total(setops_hip_batch *h, uint64_t *utotal, uint64_t *small)
 }
 
 // Tile kernels run one workgroup per tile; batches of more than kMaxGrid tiles take several
-// launches (tile offset tile0).
+// launches (tile offset tile0).  A direct plan (one plain pair in one tile, set only by the
+// single-pair entries) launches the kernels' DIRECT form, which takes the pair from the plan's
+// fields instead of the tile maps.
 template <int K, bool BSR, bool CO, bool GA, int LP>
 void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
+    if constexpr (!BSR && !GA) {
+        if (h->plan.direct) {
+            hipLaunchKernelGGL((merge_tile_kernel<kNTM, K, kMaxSeg, BSR, CO, GA, LP, true>), dim3(1), dim3(kNTM), 0,
+                               stream, h->in, h->plan, out, 0u);
+            return;
+        }
+    }
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
         hipLaunchKernelGGL((merge_tile_kernel<kNTM, K, GA ? kMaxSegGather : kMaxSeg, BSR, CO, GA, LP>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNTM), 0, stream,
@@ -508,6 +517,13 @@ void launch_merge_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 template <int KS, bool BSR, bool CO, int SMODE>
 void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream)
 {
+    if constexpr (!BSR) {
+        if (h->plan.direct) {
+            hipLaunchKernelGGL((search_tile_kernel<kNT, KS, kMaxSeg, kCapB, BSR, CO, SMODE, true>), dim3(1), dim3(kNT),
+                               0, stream, h->in, h->plan, out, 0u);
+            return;
+        }
+    }
     for (uint64_t t0 = 0; t0 < h->plan.ntiles; t0 += kMaxGrid)
         hipLaunchKernelGGL((search_tile_kernel<kNT, KS, kMaxSeg, kCapB, BSR, CO, SMODE>),
                            dim3((uint32_t)std::min<uint64_t>(kMaxGrid, h->plan.ntiles - t0)), dim3(kNT), 0, stream,
@@ -573,10 +589,9 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
                   std::pair<hipEvent_t, hipEvent_t> *ev)
 {
     // plan: unit prefix and tile maps, then tile starts (one pair over several merge tiles: both
-    // in one launch; one pair in one merge tile: none, the tile kernel takes the pair as it is)
+    // in one launch; one pair in one tile: none, the tile kernel takes the pair as it is)
     const bool single_split = h->plan.single && !h->plan.direct && h->plan.policy == POLICY_MERGE;
-    const bool direct_merge = h->plan.direct && h->plan.policy == POLICY_MERGE;
-    if (direct_merge) {
+    if (h->plan.direct) {
         // no plan launch
     } else if (single_split) {
         hipLaunchKernelGGL(plan_split_single_kernel, dim3(grid_for(h->plan.ntiles, 256)), dim3(256), 0, stream,

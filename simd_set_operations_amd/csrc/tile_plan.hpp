@@ -756,19 +756,26 @@ __device__ __forceinline__ SegWin seg_window(const PairBatch &in, const TilePlan
 }
 
 // The one segment of a direct plan (one pair in one tile): the whole pair, starting and ending
-// in the tile at unit 0.
+// in the tile at unit 0 (what seg_window gives for it under the tile kind's policy).
+template <bool SEARCH>
 __device__ __forceinline__ SegWin direct_window(const TilePlan &plan)
 {
     SegWin w;
+    const uint64_t na = plan.single_na, nb = plan.single_nb;
     w.p = 0;
     w.a = 0;
     w.b = 0;
-    w.na = plan.single_na;
-    w.nb = plan.single_nb;
+    w.na = na;
+    w.nb = nb;
     w.i0 = 0;
-    w.i1 = plan.single_na;
     w.j0 = 0;
-    w.j1 = plan.single_nb;
+    if constexpr (SEARCH) {
+        w.i1 = na < nb ? na : nb;
+        w.j1 = 0;
+    } else {
+        w.i1 = na;
+        w.j1 = nb;
+    }
     w.u = 0;
     w.flags = 3u;
     return w;
