@@ -567,8 +567,9 @@ def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
     lengths in [2^4, 2^18] over [0, 2^26)); queries of k ~ U{2..6} lists by Zipf(1.0) popularity,
     answered smallest-first, arriving as Poisson streams.  The continuous-batching engine (each
     admitted query answered whole in one batch of k-set groups, two steps in flight on two streams,
-    admission against the free HBM less a reserve) is compared with the plain path (one query at a
-    time, one launch sequence per SvS round) on the same stream.  Per-query elements follow the
+    admission against the free HBM less a reserve; its request loop in the library) is compared
+    with the same engine driven by its Python loop and with the plain path (one query at a time,
+    one launch sequence per SvS round) on the same stream.  Per-query elements follow the
     reference's convention (sum of the k list sizes)."""
     from simd_set_operations_amd import serving
 
@@ -592,6 +593,10 @@ def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
         r = {"queries": nq, "engine": serving.latency_summary(lat), "engine_steps": st["steps"],
              "engine_queries_per_step": st["pairs"] / max(st["steps"], 1), "engine_qps": st["served"] / st["wall_s"],
              "engine_gelems": E / st["wall_s"] / 1e9}
+        eng.serve(wids, wks, serving.poisson_arrivals(2000, rate, seed=400 + i), loop="python")
+        lat_p, _, st_p = eng.serve(ids, ks, arr, loop="python")
+        r["engine_python_loop"] = serving.latency_summary(lat_p)
+        r["engine_python_loop_steps"] = st_p["steps"]
         plat, _, pst = serving.plain_serve(corpus, ids, ks, arr, timeout_s=plain_budget_s)
         r["plain"] = serving.latency_summary(plat)
         r["plain_qps"] = pst["served"] / pst["wall_s"]

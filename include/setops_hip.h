@@ -233,6 +233,59 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
                           uint64_t out_cap, uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals,
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream);
 
+/* One step slot of the native request loop: a gather handle over the corpus pool and the buffers
+ * its steps use (the arguments of setops_hip_batch_step).  h_idx holds idx_words uint64 words
+ * (pinned host memory), d_idx as many on the device; d_out and h_out (pinned) out_words words;
+ * d_out_off and h_out_off (pinned) max_pairs + 1 words, d_counts max_pairs, d_totals 2. */
+typedef struct setops_hip_serve_slot {
+    setops_hip_batch *handle;
+    uint64_t *h_idx;
+    uint64_t *d_idx;
+    uint64_t idx_words;
+    uint32_t *d_out;
+    uint32_t *h_out;
+    uint64_t out_words;
+    uint64_t *d_out_off;
+    uint64_t *h_out_off;
+    uint32_t *d_counts;
+    unsigned long long *d_totals;
+    uint64_t max_pairs;
+    void *stream;
+} setops_hip_serve_slot;
+
+/* A stream of k-set queries, in arrival order.  Query q is the pair (A, B) =
+ * ([pairs[q], pairs[nq + q]), [pairs[2nq + q], pairs[3nq + q])) of pool offsets with sizes na[q],
+ * nb[q], and further sets [xb[j], xe[j]) for j in [xoff[q], xoff[q+1]).  Its result needs at most
+ * need_pfx[q+1] - need_pfx[q] words; it holds cost[q] words of the admission budget while in
+ * flight; it arrives arrivals[q] seconds after the loop starts (ascending). */
+typedef struct setops_hip_serve_queries {
+    uint64_t nq;
+    const uint64_t *pairs;
+    const uint64_t *na;
+    const uint64_t *nb;
+    const int64_t *xoff;
+    const uint64_t *xb;
+    const uint64_t *xe;
+    const int64_t *need_pfx;
+    const int64_t *cost;
+    const double *arrivals;
+} setops_hip_serve_queries;
+
+/* The request loop in native code: serves the query stream with the given step slots (two or more;
+ * a step on one runs while the host admits and launches the next on another), admitting
+ * arrived queries in arrival order while their costs fit `budget_words` and a step holds at most
+ * max_pairs queries, and a step's packed rows and results fit its slot.  Each query is answered
+ * completely in one step (setops_hip_batch_step with its further sets).  Outputs: done_t[q] =
+ * seconds from the start when query q's step was complete on the host (NaN if the loop stopped
+ * at timeout_s first); with results != NULL, query q's result at results[need_pfx[q] ..] and its
+ * size in counts[q]; stats = {steps, queries launched, queries served, wall time in ns}.  A query that fits no
+ * slot's buffers is SETOPS_ERR_CAPACITY before anything runs.  For serving loops whose host
+ * side should cost microseconds per step; the reference has no counterpart (the nearest caller
+ * is its k-set timer, harness.rs:152-172). */
+int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setops_hip_serve_queries *queries,
+                     int64_t budget_words, uint64_t max_pairs, double timeout_s, double *done_t, uint32_t *results,
+                     uint64_t *counts, uint64_t *stats);
+
 /* Runs the batch on `stream` (a hipStream_t, NULL for the HIP null stream).  Outputs:
  *   d_out / d_out_states : compacted results of all pairs in pair order (states: BSR only);
  *                          both NULL selects count-only mode (out_cap ignored)

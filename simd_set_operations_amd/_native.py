@@ -50,6 +50,25 @@ class SetopsError(RuntimeError):
         self.code = code
 
 
+class ServeSlot(ctypes.Structure):
+    """setops_hip_serve_slot: a step slot of the native request loop."""
+
+    _fields_ = [("handle", ctypes.c_void_p), ("h_idx", ctypes.c_void_p), ("d_idx", ctypes.c_void_p),
+                ("idx_words", ctypes.c_uint64), ("d_out", ctypes.c_void_p), ("h_out", ctypes.c_void_p),
+                ("out_words", ctypes.c_uint64), ("d_out_off", ctypes.c_void_p), ("h_out_off", ctypes.c_void_p),
+                ("d_counts", ctypes.c_void_p), ("d_totals", ctypes.c_void_p), ("max_pairs", ctypes.c_uint64),
+                ("stream", ctypes.c_void_p)]
+
+
+class ServeQueries(ctypes.Structure):
+    """setops_hip_serve_queries: a query stream in arrival order (host arrays)."""
+
+    _fields_ = [("nq", ctypes.c_uint64), ("pairs", ctypes.c_void_p), ("na", ctypes.c_void_p),
+                ("nb", ctypes.c_void_p), ("xoff", ctypes.c_void_p), ("xb", ctypes.c_void_p),
+                ("xe", ctypes.c_void_p), ("need_pfx", ctypes.c_void_p), ("cost", ctypes.c_void_p),
+                ("arrivals", ctypes.c_void_p)]
+
+
 # Every exported symbol of include/setops_hip.h with its ctypes signature.
 _c = ctypes
 _u32p = _c.POINTER(_c.c_uint32)
@@ -94,6 +113,7 @@ HIP_SIGNATURES = {
     "setops_hip_batch_reserve": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint64]),
     "setops_hip_batch_step": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _c.c_int64, _vp, _vp, _vp, _c.c_uint64, _vp,
                                          _vp, _vp, _vp, _vp, _c.c_uint64, _vp]),
+    "setops_hip_serve": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_int64, _c.c_uint64, _c.c_double, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_run": (_c.c_int, [_vp, _vp, _vp, _c.c_uint64, _vp, _vp, _vp, _vp]),
     "setops_hip_batch_check": (_c.c_int, [_vp]),
     "setops_hip_batch_set_timing": (_c.c_int, [_vp, _c.c_int]),

@@ -6,12 +6,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/setops_hip.h"
@@ -1469,6 +1472,175 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
     HIP_TRY(hipMemcpyAsync(h_out_off, d_out_off, (P + 1) * 8, hipMemcpyDeviceToHost, s));
     if (copy_words) HIP_TRY(hipMemcpyAsync(h_out, d_out, copy_words * 4, hipMemcpyDeviceToHost, s));
     return SETOPS_OK;
+}
+
+// ---- native request loop ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr uint64_t kStepSlackWords = 64;  // result words a step copies back beyond its queries' need
+
+// Packed step rows of query q (setops_hip_batch_step's layout): 4 pair bounds and 1 further-set
+// offset per query, 2 words per further set, plus the closing offset of the step.
+uint64_t query_rows(const setops_hip_serve_queries *qs, uint64_t q)
+{
+    return 5 + 2 * (uint64_t)(qs->xoff[q + 1] - qs->xoff[q]);
+}
+
+uint64_t query_need(const setops_hip_serve_queries *qs, uint64_t q)
+{
+    return (uint64_t)(qs->need_pfx[q + 1] - qs->need_pfx[q]);
+}
+
+struct ServeState {
+    hipEvent_t done = nullptr;
+    uint64_t q0 = 0, q1 = 0;  // queries of the step in flight on the slot (none: q0 == q1)
+    int64_t cost = 0;          // their admission cost
+};
+
+// Copies a step's packed rows for queries q0 .. q1-1 into the slot's pinned staging.
+void pack_step(const setops_hip_serve_slot &sl, const setops_hip_serve_queries *qs, uint64_t q0, uint64_t q1)
+{
+    const uint64_t P = q1 - q0, nq = qs->nq;
+    const int64_t x0 = qs->xoff[q0], X = qs->xoff[q1] - x0;
+    uint64_t *h = sl.h_idx;
+    for (int r = 0; r < 4; r++) std::memcpy(h + r * P, qs->pairs + r * nq + q0, P * 8);
+    for (uint64_t j = 0; j <= P; j++) h[4 * P + j] = (uint64_t)(qs->xoff[q0 + j] - x0);
+    std::memcpy(h + 5 * P + 1, qs->xb + x0, (size_t)X * 8);
+    std::memcpy(h + 5 * P + 1 + X, qs->xe + x0, (size_t)X * 8);
+}
+
+}  // namespace
+
+int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setops_hip_serve_queries *qs,
+                     int64_t budget_words, uint64_t max_pairs, double timeout_s, double *done_t, uint32_t *results,
+                     uint64_t *counts, uint64_t *stats)
+{
+    if (!slots || !qs || !done_t || !stats || (results && !counts)) return fail(SETOPS_ERR_INVALID, "NULL argument");
+    if (nslots < 2) return fail(SETOPS_ERR_INVALID, "the loop needs at least two slots (got %d)", nslots);
+    if (max_pairs == 0) return fail(SETOPS_ERR_INVALID, "max_pairs is 0");
+    const uint64_t nq = qs->nq;
+    if (nq && (!qs->pairs || !qs->na || !qs->nb || !qs->xoff || !qs->need_pfx || !qs->cost || !qs->arrivals ||
+               (qs->xoff[nq] > qs->xoff[0] && (!qs->xb || !qs->xe))))
+        return fail(SETOPS_ERR_INVALID, "NULL query array");
+    uint64_t idx_min = ~0ull, out_min = ~0ull;
+    for (int i = 0; i < nslots; i++) {
+        const setops_hip_serve_slot &sl = slots[i];
+        if (!sl.handle || !sl.h_idx || !sl.d_idx || !sl.d_out || !sl.h_out || !sl.d_out_off || !sl.h_out_off ||
+            !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
+            return fail(SETOPS_ERR_INVALID, "slot %d: NULL buffer or no pair slots", i);
+        idx_min = std::min(idx_min, sl.idx_words);
+        out_min = std::min(out_min, sl.out_words);
+    }
+    // every query must fit a step of its own on any slot, and the budget
+    for (uint64_t q = 0; q < nq; q++) {
+        if (qs->xoff[q + 1] < qs->xoff[q] || qs->need_pfx[q + 1] < qs->need_pfx[q] || qs->cost[q] < 0)
+            return fail(SETOPS_ERR_INVALID, "query %llu: offsets decrease or cost is negative", (unsigned long long)q);
+        if (q && qs->arrivals[q] < qs->arrivals[q - 1])
+            return fail(SETOPS_ERR_INVALID, "arrivals are not ascending at query %llu", (unsigned long long)q);
+        if (1 + query_rows(qs, q) > idx_min || query_need(qs, q) + kStepSlackWords > out_min)
+            return fail(SETOPS_ERR_CAPACITY, "query %llu does not fit a slot's step buffers", (unsigned long long)q);
+        if (qs->cost[q] > budget_words)
+            return fail(SETOPS_ERR_CAPACITY, "query %llu needs more than the admission budget", (unsigned long long)q);
+    }
+    for (uint64_t q = 0; q < nq; q++) done_t[q] = std::numeric_limits<double>::quiet_NaN();
+
+    std::vector<ServeState> st((size_t)nslots);
+    for (auto &s : st) HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    // waits for the steps still in flight (their buffers are the caller's) and frees the events
+    auto finish = [&](int rc) {
+        for (int i = 0; i < nslots; i++) {
+            if (st[i].q1 > st[i].q0) (void)hipEventSynchronize(st[i].done);
+            (void)hipEventDestroy(st[i].done);
+        }
+        return rc;
+    };
+
+    using clock = std::chrono::steady_clock;
+    const clock::time_point t0 = clock::now();
+    auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    uint64_t head = 0, nxt = 0, ndone = 0, steps = 0, launched = 0, active = 0;
+    int64_t reserved = 0;
+    int cur = 0;
+    while (ndone < nq) {
+        const double now = since();
+        if (now > timeout_s) break;
+        while (nxt < nq && qs->arrivals[nxt] <= now) nxt++;
+        const setops_hip_serve_slot &sl = slots[cur];
+        ServeState &ss = st[cur];
+        // this slot is free (its last step was finished when it was the oldest): admit the
+        // arrived queries in arrival order while they fit the budget, the pair slots and the
+        // slot's step buffers (the first that does not fit stops the admission)
+        if (head < nxt) {
+            uint64_t e = head, rows = 1, need = kStepSlackWords;
+            int64_t cost = 0;
+            while (e < nxt && active + (e - head) < max_pairs && (e - head) < sl.max_pairs) {
+                const int64_t c = qs->cost[e];
+                const uint64_t r = query_rows(qs, e), n = query_need(qs, e);
+                if (reserved + cost + c > budget_words || rows + r > sl.idx_words || need + n > sl.out_words) break;
+                cost += c;
+                rows += r;
+                need += n;
+                e++;
+            }
+            if (e > head) {
+                pack_step(sl, qs, head, e);
+                const uint64_t P = e - head;
+                const int64_t X = qs->xoff[e] - qs->xoff[head];
+                int rc = setops_hip_batch_step(sl.handle, sl.h_idx, sl.d_idx, P, X, qs->na + head, qs->nb + head,
+                                               sl.d_out, sl.out_words, sl.d_out_off, sl.d_counts, sl.d_totals,
+                                               sl.h_out_off, sl.h_out, need, sl.stream);
+                if (rc != SETOPS_OK) return finish(rc);
+                hipStream_t s = static_cast<hipStream_t>(sl.stream);
+                if (hipEventRecord(ss.done, s) != hipSuccess) return finish(fail(SETOPS_ERR_HIP, "hipEventRecord failed"));
+                ss.q0 = head;
+                ss.q1 = e;
+                ss.cost = cost;
+                reserved += cost;
+                active += P;
+                head = e;
+                steps++;
+                launched += P;
+            }
+        }
+        // the oldest step in flight: wait for it and finish its queries
+        const int old = (cur + 1) % nslots;
+        ServeState &os = st[old];
+        if (os.q1 > os.q0) {
+            if (hipEventSynchronize(os.done) != hipSuccess)
+                return finish(fail(SETOPS_ERR_HIP, "hipEventSynchronize failed"));
+            const double t_end = since();
+            const setops_hip_serve_slot &ol = slots[old];
+            for (uint64_t q = os.q0; q < os.q1; q++) {
+                done_t[q] = t_end;
+                if (results) {
+                    const uint64_t j = q - os.q0, o = ol.h_out_off[j], n = ol.h_out_off[j + 1] - o;
+                    if (ol.h_out_off[j + 1] < o || n > query_need(qs, q) || o + n > ol.out_words)
+                        return finish(fail(SETOPS_ERR_CAPACITY, "query %llu: result larger than its bound",
+                                           (unsigned long long)q));
+                    std::memcpy(results + qs->need_pfx[q], ol.h_out + o, n * 4);
+                    counts[q] = n;
+                }
+            }
+            ndone += os.q1 - os.q0;
+            active -= os.q1 - os.q0;
+            reserved -= os.cost;
+            os.q0 = os.q1 = 0;
+        } else if (head == nxt && nxt < nq &&
+                   std::all_of(st.begin(), st.end(), [](const ServeState &x) { return x.q1 == x.q0; })) {
+            // nothing in flight and nothing arrived: wait for the next arrival (sleep while it is
+            // far, spin for the last 100 us)
+            const double wait = qs->arrivals[nxt] - since();
+            if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait - 1e-4, 1e-3)));
+        }
+        cur = old;
+    }
+    const double wall = since();
+    stats[0] = steps;
+    stats[1] = launched;
+    stats[2] = ndone;
+    stats[3] = (uint64_t)(wall * 1e9);
+    return finish(SETOPS_OK);
 }
 
 int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_out_states, uint64_t out_cap,

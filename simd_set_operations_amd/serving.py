@@ -300,8 +300,77 @@ class Engine:
         sl.done.record(sl.stream)
         sl.queries = np.arange(q0, q1)
 
-    def serve(self, ids, ks, arrivals, keep_results=False, timeout_s=60.0):
-        """Serve the query stream; returns (latency seconds per query, results list or None, stats)."""
+    def serve(self, ids, ks, arrivals, keep_results=False, timeout_s=60.0, loop="native"):
+        """Serve the query stream; returns (latency seconds per query, results list or None, stats).
+
+        ``loop="native"``: the request loop runs in the library (setops_hip_serve), a few
+        microseconds of host work per step; ``loop="python"``: the same loop written here (the
+        admission, the step rows and the two slots are the same, so are the results)."""
+        if loop == "native":
+            return self._serve_native(ids, ks, arrivals, keep_results, timeout_s)
+        if loop != "python":
+            raise ValueError(f"unknown request loop {loop!r}")
+        return self._serve_python(ids, ks, arrivals, keep_results, timeout_s)
+
+    def _slot_desc(self, sl):
+        return _native.ServeSlot(
+            sl.handle.value, sl.h_idx.data_ptr(), sl.d_idx.data_ptr(), min(sl.h_idx.numel(), sl.d_idx.numel()),
+            sl.out.data_ptr(), sl.h_res.data_ptr(), min(sl.out.numel(), sl.h_res.numel()), sl.d_off.data_ptr(),
+            sl.h_off.data_ptr(), sl.d_cnt.data_ptr(), sl.d_tot.data_ptr(), self.max_pairs, sl.stream.cuda_stream)
+
+    def _serve_native(self, ids, ks, arrivals, keep_results, timeout_s):
+        import gc
+
+        nq = ids.shape[0]
+        need, cost = query_cost_words(self.c, ids)
+        budget = self.budget_bytes // 4
+        if np.any(cost > budget):
+            raise ValueError("a query needs more device memory than the serving budget")
+        rows = self._stream_rows(self.c, ids, ks, need)
+        if nq:
+            # a slot's step buffers hold at least the largest query's step (steps never grow)
+            one_rows = int((5 + 2 * np.diff(rows["xoff"])).max()) + 1
+            for sl in self.slots:
+                sl._grow(one_rows, int(need.max()) + 64)
+        self.torch.cuda.synchronize()
+        keep = []  # the host arrays the library reads, alive for the call
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        arrivals = np.ascontiguousarray(arrivals, dtype=np.float64)
+        q = _native.ServeQueries(nq, arr(rows["pairs"], np.uint64), arr(rows["na"], np.uint64),
+                                 arr(rows["nb"], np.uint64), arr(rows["xoff"], np.int64), arr(rows["xb"], np.uint64),
+                                 arr(rows["xe"], np.uint64), arr(rows["need_pfx"], np.int64), arr(cost, np.int64),
+                                 arr(arrivals, np.float64))
+        slots = (_native.ServeSlot * len(self.slots))(*[self._slot_desc(sl) for sl in self.slots])
+        done_t = np.full(nq, np.nan)
+        st = np.zeros(4, np.uint64)
+        res = np.empty(max(int(rows["need_pfx"][-1]), 1), np.uint32) if keep_results else None
+        counts = np.zeros(max(nq, 1), np.uint64) if keep_results else None
+        gc_was = gc.isenabled()
+        gc.collect()
+        gc.disable()
+        try:
+            _native.check(self.lib.setops_hip_serve(
+                _c.byref(slots), len(self.slots), _c.byref(q), budget, self.max_pairs, float(timeout_s),
+                done_t.ctypes.data, res.ctypes.data if keep_results else None,
+                counts.ctypes.data if keep_results else None, st.ctypes.data))
+        finally:
+            if gc_was:
+                gc.enable()
+        results = None
+        if keep_results:
+            pfx = rows["need_pfx"]
+            results = [res[int(pfx[i]):int(pfx[i]) + int(counts[i])].copy() if np.isfinite(done_t[i]) else None
+                       for i in range(nq)]
+        stats = {"steps": int(st[0]), "pairs": int(st[1]), "served": int(st[2]), "wall_s": float(st[3]) / 1e9,
+                 "budget_bytes": self.budget_bytes}
+        return done_t - arrivals, results, stats
+
+    def _serve_python(self, ids, ks, arrivals, keep_results, timeout_s):
         import gc
 
         nq = ids.shape[0]

@@ -128,5 +128,47 @@ def test_argument_errors_before_device_use():
                                      ctypes.byref(n)) == _native.SETOPS_ERR_UNKNOWN_ALGO
 
 
+def test_serve_loop_argument_errors_before_device_use():
+    """The native request loop checks its slots and its query stream before any device work."""
+    lib = _native.hip()
+    fake = 16  # never dereferenced: every case below fails validation first
+    sl = _native.ServeSlot(fake, fake, fake, 64, fake, fake, 128, fake, fake, fake, fake, 4, None)
+    slots = (_native.ServeSlot * 2)(sl, sl)
+    pairs = np.zeros(4, np.uint64)
+    na = np.ones(1, np.uint64)
+    xoff = np.zeros(2, np.int64)
+    need = np.array([0, 8], np.int64)
+    cost = np.array([40], np.int64)
+    arr = np.zeros(1, np.float64)
+    q = _native.ServeQueries(1, pairs.ctypes.data, na.ctypes.data, na.ctypes.data, xoff.ctypes.data, None, None,
+                             need.ctypes.data, cost.ctypes.data, arr.ctypes.data)
+    done = np.zeros(1)
+    st = np.zeros(4, np.uint64)
+
+    def serve(nslots, budget=1000, slots_=slots, q_=q):
+        return lib.setops_hip_serve(ctypes.byref(slots_), nslots, ctypes.byref(q_), budget, 4, 1.0, done.ctypes.data,
+                                    None, None, st.ctypes.data)
+
+    assert serve(1) == _native.SETOPS_ERR_INVALID  # two slots at least
+    assert b"two slots" in lib.setops_hip_last_error()
+    assert lib.setops_hip_serve(None, 2, ctypes.byref(q), 1000, 4, 1.0, done.ctypes.data, None, None,
+                                st.ctypes.data) == _native.SETOPS_ERR_INVALID
+    assert serve(2, budget=39) == _native.SETOPS_ERR_CAPACITY  # the query's cost exceeds the budget
+    need[1] = 1 << 20  # a result bound larger than the slots' result buffers
+    assert serve(2) == _native.SETOPS_ERR_CAPACITY
+    need[1] = 8
+    bad = _native.ServeSlot(None, fake, fake, 64, fake, fake, 128, fake, fake, fake, fake, 4, None)
+    assert serve(2, slots_=(_native.ServeSlot * 2)(sl, bad)) == _native.SETOPS_ERR_INVALID
+    arr2 = np.array([1.0, 0.5])  # arrivals must ascend
+    pairs2, na2, xoff2 = np.zeros(8, np.uint64), np.ones(2, np.uint64), np.zeros(3, np.int64)
+    need2, cost2 = np.array([0, 8, 16], np.int64), np.array([40, 40], np.int64)
+    q2 = _native.ServeQueries(2, pairs2.ctypes.data, na2.ctypes.data, na2.ctypes.data, xoff2.ctypes.data, None, None,
+                              need2.ctypes.data, cost2.ctypes.data, arr2.ctypes.data)
+    done2 = np.zeros(2)
+    assert lib.setops_hip_serve(ctypes.byref(slots), 2, ctypes.byref(q2), 1000, 4, 1.0, done2.ctypes.data, None, None,
+                                st.ctypes.data) == _native.SETOPS_ERR_INVALID
+    assert b"ascending" in lib.setops_hip_last_error()
+
+
 def test_version_string():
     assert b"gfx950" in _native.hip().setops_hip_version()

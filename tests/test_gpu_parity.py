@@ -587,7 +587,8 @@ def test_graph_triangle_sums():
 # ---- served k-set queries -------------------------------------------------------------------------
 def test_served_queries_alone_and_in_the_mix():
     """Continuous batching: every query's result equals the oracle's SvS over its lists, the plain
-    per-query path's result, and its result when served alone."""
+    per-query path's result, its result when served alone, and the Python request loop's result
+    (the native loop, setops_hip_serve, is the default)."""
     from simd_set_operations_amd import serving
 
     corpus = serving.Corpus(nlists=4096, lmin=16, lmax=1 << 12, universe=1 << 18, seed=77)
@@ -619,6 +620,19 @@ def test_served_queries_alone_and_in_the_mix():
     for q in range(0, ids.shape[0], 100):  # alone: a stream of one query
         _, r1, _ = eng.serve(ids[q:q + 1], ks[q:q + 1], np.zeros(1), keep_results=True)
         assert np.array_equal(r1[0], res[q]), q
+    announce("serving python loop")
+    _, rp, sp = eng.serve(ids, ks, arrivals, keep_results=True, loop="python")
+    assert sp["served"] == ids.shape[0]
+    for q in range(ids.shape[0]):
+        assert np.array_equal(rp[q], res[q]), q
+    # all at once (every query pending from the start: steps of max_pairs queries), then a stream
+    # stopped by its time limit before anything is served (a host-side condition)
+    _, ra, sa = eng.serve(ids, ks, np.zeros(ids.shape[0]), keep_results=True)
+    assert sa["served"] == ids.shape[0] and sa["steps"] >= ids.shape[0] // 256
+    for q in range(ids.shape[0]):
+        assert np.array_equal(ra[q], res[q]), q
+    lat0, r0, s0 = eng.serve(ids, ks, arrivals + 10.0, keep_results=True, timeout_s=0.0)
+    assert s0["served"] == 0 and s0["steps"] == 0 and np.all(np.isnan(lat0)) and all(r is None for r in r0)
     eng.close()
 
 
