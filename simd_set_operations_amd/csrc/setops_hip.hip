@@ -1450,7 +1450,7 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream)
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
-    if (!h_idx || !d_idx || !d_out_off || !h_out_off || (copy_words && (!h_out || !d_out)))
+    if (!h_idx || !d_idx || !d_out_off || (copy_words && (!h_out || !d_out)))
         return fail(SETOPS_ERR_INVALID, "NULL argument");
     if (copy_words > out_cap) return fail(SETOPS_ERR_INVALID, "copy_words exceeds out_cap");
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
@@ -1469,7 +1469,7 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
     if (rc != SETOPS_OK) return rc;
     rc = batch_launch(handle, d_out, nullptr, out_cap, d_out_off, d_counts, d_totals, s);
     if (rc != SETOPS_OK) return rc;
-    HIP_TRY(hipMemcpyAsync(h_out_off, d_out_off, (P + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (h_out_off) HIP_TRY(hipMemcpyAsync(h_out_off, d_out_off, (P + 1) * 8, hipMemcpyDeviceToHost, s));
     if (copy_words) HIP_TRY(hipMemcpyAsync(h_out, d_out, copy_words * 4, hipMemcpyDeviceToHost, s));
     return SETOPS_OK;
 }
@@ -1496,6 +1496,11 @@ struct ServeState {
     hipEvent_t done = nullptr;
     uint64_t q0 = 0, q1 = 0;  // queries of the step in flight on the slot (none: q0 == q1)
     int64_t cost = 0;          // their admission cost
+    // the step's offsets and results, written by the kernels straight into mapped host memory
+    // (no copies back): max_pairs + 1 offsets, then out_words result words
+    void *host = nullptr;
+    uint64_t *h_off = nullptr, *d_off = nullptr;
+    uint32_t *h_res = nullptr, *d_res = nullptr;
 };
 
 // Copies a step's packed rows for queries q0 .. q1-1 into the slot's pinned staging.
@@ -1546,15 +1551,36 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
     for (uint64_t q = 0; q < nq; q++) done_t[q] = std::numeric_limits<double>::quiet_NaN();
 
     std::vector<ServeState> st((size_t)nslots);
-    for (auto &s : st) HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-    // waits for the steps still in flight (their buffers are the caller's) and frees the events
+    // waits for the steps still in flight and frees the events and the mapped buffers
     auto finish = [&](int rc) {
         for (int i = 0; i < nslots; i++) {
             if (st[i].q1 > st[i].q0) (void)hipEventSynchronize(st[i].done);
-            (void)hipEventDestroy(st[i].done);
+            if (st[i].done) (void)hipEventDestroy(st[i].done);
+            if (st[i].host) (void)hipHostFree(st[i].host);
         }
         return rc;
     };
+    for (int i = 0; i < nslots; i++) {
+        ServeState &s = st[i];
+        if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+            s.done = nullptr;
+            return finish(fail(SETOPS_ERR_HIP, "hipEventCreateWithFlags failed"));
+        }
+        const size_t off_bytes = ((size_t)(slots[i].max_pairs + 1) * 8 + 255) & ~(size_t)255;
+        const size_t bytes = off_bytes + (size_t)slots[i].out_words * 4;
+        if (hipHostMalloc(&s.host, bytes, hipHostMallocMapped) != hipSuccess) {
+            (void)hipGetLastError();
+            s.host = nullptr;
+            return finish(fail(SETOPS_ERR_NOMEM, "hipHostMalloc of %zu mapped bytes failed", bytes));
+        }
+        void *dev = nullptr;
+        if (hipHostGetDevicePointer(&dev, s.host, 0) != hipSuccess || !dev)
+            return finish(fail(SETOPS_ERR_HIP, "hipHostGetDevicePointer failed"));
+        s.h_off = static_cast<uint64_t *>(s.host);
+        s.d_off = static_cast<uint64_t *>(dev);
+        s.h_res = reinterpret_cast<uint32_t *>(static_cast<char *>(s.host) + off_bytes);
+        s.d_res = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + off_bytes);
+    }
 
     using clock = std::chrono::steady_clock;
     const clock::time_point t0 = clock::now();
@@ -1588,8 +1614,8 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
                 const uint64_t P = e - head;
                 const int64_t X = qs->xoff[e] - qs->xoff[head];
                 int rc = setops_hip_batch_step(sl.handle, sl.h_idx, sl.d_idx, P, X, qs->na + head, qs->nb + head,
-                                               sl.d_out, sl.out_words, sl.d_out_off, sl.d_counts, sl.d_totals,
-                                               sl.h_out_off, sl.h_out, need, sl.stream);
+                                               ss.d_res, sl.out_words, ss.d_off, sl.d_counts, sl.d_totals,
+                                               nullptr, nullptr, 0, sl.stream);
                 if (rc != SETOPS_OK) return finish(rc);
                 hipStream_t s = static_cast<hipStream_t>(sl.stream);
                 if (hipEventRecord(ss.done, s) != hipSuccess) return finish(fail(SETOPS_ERR_HIP, "hipEventRecord failed"));
@@ -1614,11 +1640,11 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
             for (uint64_t q = os.q0; q < os.q1; q++) {
                 done_t[q] = t_end;
                 if (results) {
-                    const uint64_t j = q - os.q0, o = ol.h_out_off[j], n = ol.h_out_off[j + 1] - o;
-                    if (ol.h_out_off[j + 1] < o || n > query_need(qs, q) || o + n > ol.out_words)
+                    const uint64_t j = q - os.q0, o = os.h_off[j], n = os.h_off[j + 1] - o;
+                    if (os.h_off[j + 1] < o || n > query_need(qs, q) || o + n > ol.out_words)
                         return finish(fail(SETOPS_ERR_CAPACITY, "query %llu: result larger than its bound",
                                            (unsigned long long)q));
-                    std::memcpy(results + qs->need_pfx[q], ol.h_out + o, n * 4);
+                    std::memcpy(results + qs->need_pfx[q], os.h_res + o, n * 4);
                     counts[q] = n;
                 }
             }
