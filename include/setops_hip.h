@@ -225,28 +225,27 @@ int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t
  * then, with nfurther >= 0, x_off[P+1], x_beg[nfurther], x_end[nfurther]; h_idx in pinned host
  * memory), rebinds the handle to them (h_na / h_nb: the pairs' sizes, as for
  * setops_hip_batch_rebind), attaches the further sets (nfurther < 0: none), runs it with outputs
- * (as setops_hip_batch_run), then enqueues the copies of d_out_off[0 .. P] to h_out_off and of
- * d_out[0 .. copy_words) to h_out.  For serving loops that answer new queries every step; the
+ * (as setops_hip_batch_run), then enqueues the copies of d_out_off[0 .. P] to h_out_off (none if
+ * h_out_off is NULL, e.g. for offsets in mapped host memory) and of d_out[0 .. copy_words) to
+ * h_out.  For serving loops that answer new queries every step; the
  * reference has no counterpart. */
 int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint64_t *d_idx, uint64_t npairs,
                           int64_t nfurther, const uint64_t *h_na, const uint64_t *h_nb, uint32_t *d_out,
                           uint64_t out_cap, uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals,
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream);
 
-/* One step slot of the native request loop: a gather handle over the corpus pool and the buffers
- * its steps use (the arguments of setops_hip_batch_step).  h_idx holds idx_words uint64 words
- * (pinned host memory), d_idx as many on the device; d_out and h_out (pinned) out_words words;
- * d_out_off and h_out_off (pinned) max_pairs + 1 words, d_counts max_pairs, d_totals 2. */
+/* One step slot of the native request loop: a gather handle over the corpus pool, the staging of
+ * its steps' packed rows (h_idx: idx_words uint64 words of pinned host memory, d_idx as many on the
+ * device), its per-pair counts (d_counts, max_pairs words) and totals (d_totals, 2 words), and the
+ * result words a step may hold (out_words).  A step's offsets and results are written by its
+ * kernels straight into host memory that the loop maps for the device (one buffer per slot,
+ * allocated when the loop starts): there are no copies back. */
 typedef struct setops_hip_serve_slot {
     setops_hip_batch *handle;
     uint64_t *h_idx;
     uint64_t *d_idx;
     uint64_t idx_words;
-    uint32_t *d_out;
-    uint32_t *h_out;
     uint64_t out_words;
-    uint64_t *d_out_off;
-    uint64_t *h_out_off;
     uint32_t *d_counts;
     unsigned long long *d_totals;
     uint64_t max_pairs;

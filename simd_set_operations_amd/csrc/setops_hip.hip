@@ -1478,7 +1478,7 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
 
 namespace {
 
-constexpr uint64_t kStepSlackWords = 64;  // result words a step copies back beyond its queries' need
+constexpr uint64_t kStepSlackWords = 64;  // result words a step holds beyond its queries' need
 
 // Packed step rows of query q (setops_hip_batch_step's layout): 4 pair bounds and 1 further-set
 // offset per query, 2 words per further set, plus the closing offset of the step.
@@ -1531,8 +1531,7 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
     uint64_t idx_min = ~0ull, out_min = ~0ull;
     for (int i = 0; i < nslots; i++) {
         const setops_hip_serve_slot &sl = slots[i];
-        if (!sl.handle || !sl.h_idx || !sl.d_idx || !sl.d_out || !sl.h_out || !sl.d_out_off || !sl.h_out_off ||
-            !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
+        if (!sl.handle || !sl.h_idx || !sl.d_idx || !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
             return fail(SETOPS_ERR_INVALID, "slot %d: NULL buffer or no pair slots", i);
         idx_min = std::min(idx_min, sl.idx_words);
         out_min = std::min(out_min, sl.out_words);
