@@ -842,9 +842,12 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
             }
             plan.tile_count[t] = tot;
             plan.tile_elems[t] = el;
-            if (plan.direct && out.totals) {  // one tile: its counts are the totals
-                out.totals[0] = tot;
-                out.totals[1] = el;
+            if (plan.one_tile) {  // one tile: its counts are the totals, its results the output
+                if (out.totals) {
+                    out.totals[0] = tot;
+                    out.totals[1] = el;
+                }
+                if (out.out_off) out.out_off[in.P] = tot;
             }
         }
         if (tid < nseg) {
@@ -858,6 +861,7 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
             const uint32_t f = sm.seg_flags[tid];
             const uint64_t p = (uint64_t)p_first + tid;
             if (f & 1u) plan.pair_local[p] = c0v;
+            if (plan.one_tile && out.out_off) out.out_off[p] = c0v;  // every pair lies in the one tile
             if (out.counts) {
                 if ((f & 3u) == 3u) out.counts[p] = c;
                 else if (c) atomicAdd(&out.counts[p], c);

@@ -625,7 +625,7 @@ int batch_enqueue(setops_hip_batch *h, const TileOut &out, bool count_only, hipS
     }
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev->second, stream));
-    if (h->plan.direct) return SETOPS_OK;  // results and totals are final
+    if (h->plan.one_tile) return SETOPS_OK;  // results, offsets and totals are final
     // scan: with few tiles the placement kernel scans the tile counts itself; with few groups one
     // kernel (a workgroup adds the earlier groups' tile counts itself); otherwise group sums,
     // one-block scan of the group sums (writes the totals), per-group scan
@@ -705,10 +705,16 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.error = h->d_error;
     // a run on another stream than the last one would share the workspace and staging with it
     if (h->last_stream && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+    // one tile of a plain search-policy batch with room for every unit's result: the tile's slot is
+    // the output and the tile kernel writes the offsets and totals (no scan, no placement)
+    h->plan.one_tile = (h->plan.direct || (!count_only && h->plan.ntiles == 1 && !h->bsr &&
+                                           h->plan.policy == POLICY_SEARCH && cap >= h->plan.utotal))
+                           ? 1u
+                           : 0u;
     if (!count_only) {
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;
-        if (h->plan.direct) h->plan.stage = d_out;  // the one tile's slot is the output
+        if (h->plan.one_tile) h->plan.stage = d_out;  // the one tile's slot is the output
     }
     h->last_stream = stream;
     if (h->timing || !h->use_graph) {
@@ -807,6 +813,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
     h->in.contig = gather ? 0u : 1u;
     h->plan.single = 0;
     h->plan.direct = 0;
+    h->plan.one_tile = 0;
     int rc;
     const uint32_t pad = pad_units(h->kind, h->bsr, h->gather, h->shape);
     if (!gather && merge_like(e->kind)) {
@@ -983,6 +990,7 @@ int noalloc_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na,
     h->in.sflip = sflip_of(algo);
     const bool count_only = (dO == nullptr && dOs == nullptr);
     h->plan.direct = (ntiles == 1 && !e->bsr && (count_only || cap >= std::min(na, nb))) ? 1u : 0u;
+    h->plan.one_tile = h->plan.direct;
     if (!count_only && h->plan.direct) h->plan.stage = dO;  // the one tile's slot is the output
     TileOut out{};
     out.out = dO;

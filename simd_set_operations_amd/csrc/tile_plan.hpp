@@ -107,6 +107,10 @@ struct TilePlan {
     // single pair in one tile, plain sets, output capacity >= min(na, nb): `stage` is the caller's
     // output, the tile kernel writes the totals, and the split, scan and placement are skipped
     uint32_t direct;
+    // the one tile's outputs are final (a direct plan, or a plain search-policy batch of one tile
+    // with output room for every unit): `stage` is the caller's output and the search tile writes
+    // the per-pair offsets and the totals; the scan and placement are skipped
+    uint32_t one_tile;
 };
 
 struct TileOut {

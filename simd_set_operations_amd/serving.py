@@ -228,7 +228,7 @@ class Engine:
     lists and further sets with about one 128-byte line per lookup."""
 
     def __init__(self, corpus, algo="hip_galloping", max_pairs=16384, index=True, reserve_bytes=16 << 30,
-                 budget_bytes=None):
+                 budget_bytes=None, nslots=2):
         torch = corpus.torch
         self.c = corpus
         self.torch = torch
@@ -241,7 +241,11 @@ class Engine:
         if budget_bytes <= 0:
             raise ValueError(f"no device memory left for serving (free HBM minus a {reserve_bytes} byte reserve)")
         self.budget_bytes = int(budget_bytes)
-        self.slots = [_Slot(self), _Slot(self)]
+        if nslots < 2:
+            raise ValueError("the engine needs at least two step slots")
+        # the native loop takes any number of slots (steps launched round-robin, the host waiting
+        # for the oldest); the Python loop alternates between the first two
+        self.slots = [_Slot(self) for _ in range(int(nslots))]
         if index:
             # every pair and every further set reads corpus lists, which never change
             for sl in self.slots:

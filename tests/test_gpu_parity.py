@@ -634,6 +634,13 @@ def test_served_queries_alone_and_in_the_mix():
     lat0, r0, s0 = eng.serve(ids, ks, arrivals + 10.0, keep_results=True, timeout_s=0.0)
     assert s0["served"] == 0 and s0["steps"] == 0 and np.all(np.isnan(lat0)) and all(r is None for r in r0)
     eng.close()
+    announce("serving three slots, 64 pairs per step")
+    eng3 = serving.Engine(corpus, "hip_galloping", max_pairs=64, nslots=3)
+    _, r3, s3 = eng3.serve(ids, ks, arrivals, keep_results=True)
+    assert s3["served"] == ids.shape[0] and s3["pairs"] == ids.shape[0]
+    for q in range(ids.shape[0]):
+        assert np.array_equal(r3[q], res[q]), q
+    eng3.close()
 
 
 def test_batch_graph_replays_match():
