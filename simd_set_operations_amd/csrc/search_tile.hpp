@@ -434,25 +434,38 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
         const uint32_t t = tile0 + blockIdx.x;
         if (t >= plan.ntiles) return;
         // ---- 1. segment table, probe windows -------------------------------------------------------
-        // DIRECT (one pair in one tile, the launch for plan.direct): nothing planned the tile; the
-        // pair is its one segment
-        static_assert(!DIRECT || !BSR, "a direct tile is one plain pair");
+        // DIRECT (the launch for plan.direct: one pair, or a batch whose pairs all lie in one tile):
+        // nothing planned the tile; each pair is one whole segment
+        static_assert(!DIRECT || !BSR, "a direct tile holds plain pairs");
         constexpr bool direct = DIRECT;
         if (direct && tid == 0 && out.error) *out.error = 0;  // before thread 0's own flags below
-        const uint32_t p_first = direct ? 0u : plan.tile_first[t], p_last = direct ? 0u : plan.tile_last[t];
+        const uint32_t p_first = direct ? 0u : plan.tile_first[t];
+        const uint32_t p_last = direct ? (plan.single ? 0u : (uint32_t)(in.P - 1)) : plan.tile_last[t];
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
         uint32_t nseg = p_last - p_first + 1;
         bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
         if (t == 0 && tid == 0 && out.error && !direct && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
+        // a direct batch: pair p's first unit is the sum of the units (probes and padding) of the
+        // pairs before it (a block scan; plan.single is uniform, so every thread takes it or none)
+        uint32_t du = 0;
+        if constexpr (DIRECT) {
+            if (!plan.single) {
+                const uint32_t v = (tid < nseg) ? (uint32_t)pair_units(in, plan.policy, plan.c0, tid) : 0u;
+                uint32_t tu = 0;
+                du = block_exclusive_scan<NT>(v, sm.scan_scratch, &tu);
+                __syncthreads();  // the scratch is scanned again below
+            }
+        }
         uint32_t cp = 0;
         uint64_t gp = 0;
         uint32_t swap = 0;
         // the span of the tile's larger B sets, for the top level of the B-pool index (1b)
         unsigned long long gmin = ~0ull, gmax = 0;
         if (tid < nseg) {
-            const SegWin w = direct ? direct_window<true>(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            const SegWin w = direct ? (plan.single ? direct_window<true>(plan) : search_pair_window(in, tid, du))
+                                    : seg_window(in, plan, t, p_first, nseg, tid, ulen);
             swap = (w.nb < w.na) ? 1u : 0u;  // probes come from the smaller side
             gp = (swap ? w.b : w.a) + w.i0;
             const uint32_t n = (uint32_t)(w.i1 - w.i0);

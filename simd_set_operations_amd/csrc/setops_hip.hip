@@ -706,11 +706,14 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     // a run on another stream than the last one would share the workspace and staging with it
     if (h->last_stream && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
     // one tile of a plain search-policy batch with room for every unit's result: the tile's slot is
-    // the output and the tile kernel writes the offsets and totals (no scan, no placement)
-    h->plan.one_tile = (h->plan.direct || (!count_only && h->plan.ntiles == 1 && !h->bsr &&
-                                           h->plan.policy == POLICY_SEARCH && cap >= h->plan.utotal))
-                           ? 1u
-                           : 0u;
+    // the output and the tile kernel writes the offsets and totals (no scan, no placement); its
+    // direct form takes the pairs as they are (no plan launch either)
+    if (!h->plan.single)
+        h->plan.direct = (!count_only && h->plan.ntiles == 1 && !h->bsr && h->plan.policy == POLICY_SEARCH &&
+                          cap >= h->plan.utotal && h->P <= (uint64_t)kMaxSeg)
+                             ? 1u
+                             : 0u;
+    h->plan.one_tile = h->plan.direct;
     if (!count_only) {
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;

@@ -785,6 +785,25 @@ __device__ __forceinline__ SegWin direct_window(const TilePlan &plan)
     return w;
 }
 
+// Pair p as one whole segment of a direct search batch (all its pairs in one tile, nothing
+// planned), starting at tile unit u: probes from the smaller side, all of them in this tile.
+__device__ __forceinline__ SegWin search_pair_window(const PairBatch &in, uint32_t p, uint32_t u)
+{
+    SegWin w;
+    w.p = p;
+    w.a = in.a_beg[p];
+    w.b = in.b_beg[p];
+    w.na = in.a_end[p] - w.a;
+    w.nb = in.b_end[p] - w.b;
+    w.i0 = 0;
+    w.i1 = w.na < w.nb ? w.na : w.nb;
+    w.j0 = 0;
+    w.j1 = 0;
+    w.u = u;
+    w.flags = 3u;
+    return w;
+}
+
 // One 16-byte chunk of a pool at element index e (16-byte aligned), guarding the pool end.
 __device__ __forceinline__ uint4 load_chunk(const uint32_t *__restrict__ src, uint64_t pool, uint64_t e)
 {
