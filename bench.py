@@ -597,10 +597,11 @@ def serving_leg(torch, rates, seconds_per_rate, plain_budget_s, mode):
         lat_p, _, st_p = eng.serve(ids, ks, arr, loop="python")
         r["engine_python_loop"] = serving.latency_summary(lat_p)
         r["engine_python_loop_steps"] = st_p["steps"]
-        plat, _, pst = serving.plain_serve(corpus, ids, ks, arr, timeout_s=plain_budget_s)
-        r["plain"] = serving.latency_summary(plat)
-        r["plain_qps"] = pst["served"] / pst["wall_s"]
-        r["plain_saturated"] = pst["served"] < nq
+        if plain_budget_s > 0:
+            plat, _, pst = serving.plain_serve(corpus, ids, ks, arr, timeout_s=plain_budget_s)
+            r["plain"] = serving.latency_summary(plat)
+            r["plain_qps"] = pst["served"] / pst["wall_s"]
+            r["plain_saturated"] = pst["served"] < nq
         res[f"{int(rate)}qps"] = r
         log(f"serving {int(rate)} qps: {r}")
     eng.close()
@@ -632,6 +633,8 @@ def main():
     ap.add_argument("--no-bsr", action="store_true", help="skip the BSR form of the uneven batch")
     ap.add_argument("--no-pair", action="store_true", help="skip the single plain pair (2^16 + 2^16)")
     ap.add_argument("--serving-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
+    ap.add_argument("--serving-plain-seconds", type=float, default=5.0,
+                    help="time limit of the plain one-query-at-a-time path per rate (0: not run)")
     ap.add_argument("--corpus-mode", choices=("iid", "rank"), default="iid")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the reference kernel of "
@@ -781,7 +784,8 @@ def main():
     if not args.no_kset:
         extra["kset"] = kset_leg(not args.no_baselines)
     if not args.no_serving:
-        extra["serving"] = serving_leg(torch, (5e4, 2e5, 8e5), args.serving_seconds, 5.0, args.corpus_mode)
+        extra["serving"] = serving_leg(torch, (5e4, 2e5, 8e5), args.serving_seconds, args.serving_plain_seconds,
+                                       args.corpus_mode)
     if not args.no_graph:
         extra["graph"] = graph_leg(torch, host_threads(), not args.no_baselines, args.graph_scale)
     if not args.no_skewed:
