@@ -455,6 +455,8 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
                 const uint32_t v = (tid < nseg) ? (uint32_t)pair_units(in, plan.policy, plan.c0, tid) : 0u;
                 uint32_t tu = 0;
                 du = block_exclusive_scan<NT>(v, sm.scan_scratch, &tu);
+                // the device sizes must give the units the host planned for (as the plan checks)
+                if (tid == 0 && out.error && (uint64_t)tu != plan.utotal) atomicOr(out.error, 4u);
                 __syncthreads();  // the scratch is scanned again below
             }
         }

@@ -261,7 +261,8 @@ struct setops_hip_batch {
     uint32_t *b_mid = nullptr;  // its middle level: every kMidStride-th element
     uint64_t stage_tiles = 0;
     unsigned long long *d_tot = nullptr;  // [2] scratch totals (unit total, or totals when the caller passes none)
-    hipStream_t last_stream = nullptr;
+    hipStream_t last_stream = nullptr;  // the stream of the last run (NULL: the HIP null stream)
+    bool ran = false;                   // a run was enqueued (on last_stream)
     bool timing = false;
     // captured launch sequence (batch_launch)
     bool use_graph = true;
@@ -339,7 +340,7 @@ int batch_reserve(setops_hip_batch *h, uint64_t P, uint64_t utotal, uint32_t c0)
     if (P >= 0xFFFFFFFFull) return fail(SETOPS_ERR_INVALID, "batch too large: %llu pairs", (unsigned long long)P);
     if (!h->ws || P > h->cap_pairs || ntiles > h->cap_tiles) {
         if (h->ws) {
-            if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+            if (h->ran) HIP_TRY(hipStreamSynchronize(h->last_stream));
             HIP_TRY(hipFree(h->ws));
             h->ws = nullptr;
         }
@@ -427,7 +428,7 @@ int stage_grow(setops_hip_batch *h, uint64_t nt)
         return SETOPS_OK;
     }
     if (h->stage) {
-        if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+        if (h->ran) HIP_TRY(hipStreamSynchronize(h->last_stream));
         HIP_TRY(hipFree(h->stage));
         h->stage = nullptr;
         h->stage_tiles = 0;
@@ -704,13 +705,13 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.totals = d_totals;
     out.error = h->d_error;
     // a run on another stream than the last one would share the workspace and staging with it
-    if (h->last_stream && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
-    // one tile of a plain search-policy batch with room for every unit's result: the tile's slot is
-    // the output and the tile kernel writes the offsets and totals (no scan, no placement); its
+    if (h->ran && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+    // one tile of a plain search-policy batch with room for every probe's result: the tile's slot
+    // is the output and the tile kernel writes the offsets and totals (no scan, no placement); its
     // direct form takes the pairs as they are (no plan launch either)
     if (!h->plan.single)
         h->plan.direct = (!count_only && h->plan.ntiles == 1 && !h->bsr && h->plan.policy == POLICY_SEARCH &&
-                          cap >= h->plan.utotal && h->P <= (uint64_t)kMaxSeg)
+                          h->P <= (uint64_t)kMaxSeg && cap >= h->plan.utotal - h->P * h->plan.c0)
                              ? 1u
                              : 0u;
     h->plan.one_tile = h->plan.direct;
@@ -720,6 +721,7 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
         if (h->plan.one_tile) h->plan.stage = d_out;  // the one tile's slot is the output
     }
     h->last_stream = stream;
+    h->ran = true;
     if (h->timing || !h->use_graph) {
         std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
         if (h->timing) {
@@ -842,7 +844,7 @@ int batch_create(setops_hip_batch **handle, int algo, const uint32_t *d_a, const
 void batch_free(setops_hip_batch *h)
 {
     if (!h) return;
-    if (h->last_stream) (void)hipStreamSynchronize(h->last_stream);
+    if (h->ran) (void)hipStreamSynchronize(h->last_stream);
     for (auto &e : h->events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -898,7 +900,7 @@ int single_launch(int algo, const uint32_t *dA, const uint32_t *dAs, size_t na, 
         h->bsr = e->bsr;
     }
     h->in.sflip = sflip_of(algo);
-    if (h->last_stream && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
+    if (h->ran && h->last_stream != s) HIP_TRY(hipStreamSynchronize(h->last_stream));
     h->P = 1;
     const uint64_t units =
         (merge_like(e->kind) ? (uint64_t)na + nb : (uint64_t)std::min(na, nb)) + pad_units(e->kind, e->bsr, false, h->shape);
@@ -1355,7 +1357,7 @@ int setops_hip_batch_index_b(setops_hip_batch *handle)
     // the B pool may have been written by work still pending on another stream; the index build
     // waits for it (it synchronises at the end anyway)
     HIP_TRY(hipDeviceSynchronize());
-    hipStream_t s = handle->last_stream ? handle->last_stream : g_ctx.stream;
+    hipStream_t s = handle->ran ? handle->last_stream : g_ctx.stream;
     const uint32_t f = handle->in.sflip;  // the index holds values in the compare space
     hipLaunchKernelGGL(build_index_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, handle->in.B, handle->in.b_pool,
                        handle->b_idx, (uint64_t)32, f);
@@ -1438,7 +1440,7 @@ int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t
         // ws_layout repoints the plan's arrays; every run rewrites them, and a cached launch graph
         // keyed on the old pointers is not replayed
         if (h->ws) {
-            if (h->last_stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
+            if (h->ran) HIP_TRY(hipStreamSynchronize(h->last_stream));
             HIP_TRY(hipFree(h->ws));
             h->ws = nullptr;
         }
@@ -1690,7 +1692,7 @@ int setops_hip_batch_run(setops_hip_batch *handle, uint32_t *d_out, uint32_t *d_
 int setops_hip_batch_check(setops_hip_batch *handle)
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
-    if (!handle->last_stream) return SETOPS_OK;
+    if (!handle->ran) return SETOPS_OK;
     uint32_t err = 0;
     HIP_TRY(hipMemcpyAsync(&err, handle->d_error, 4, hipMemcpyDeviceToHost, handle->last_stream));
     HIP_TRY(hipStreamSynchronize(handle->last_stream));

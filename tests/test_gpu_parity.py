@@ -349,6 +349,46 @@ def test_batch_reserve_keeps_results(name):
     gb.close()
 
 
+@pytest.mark.parametrize("P", [3, 2000])
+def test_rebind_sizes_that_differ_from_the_device_are_reported(P):
+    """Host sizes given to setops_hip_batch_rebind that make fewer units than the device offsets
+    are reported by setops_hip_batch_check: in a batch of one search tile (whose direct form
+    compares the units it scans with the plan's) and in a batch of many tiles (the plan's check);
+    consistent sizes again give the oracle's results."""
+    from simd_set_operations_amd import _native
+    from simd_set_operations_amd.batch import GatherBatch
+
+    rng = np.random.Generator(np.random.PCG64(95))
+    sets = [sorted_set(rng, int(rng.integers(8, 60)), 4000) for _ in range(50)]
+    pool, off = _csr(sets)
+    ia, ib = rng.integers(0, 50, size=P), rng.integers(0, 50, size=P)
+    announce(f"rebind size mismatch P={P}")
+    gb = GatherBatch("hip_galloping", pool, off[ia], off[ia + 1], pool, off[ib], off[ib + 1])
+    assert (gb.info()["tiles"] == 1) == (P == 3)
+    lib = _native.hip()
+    na = (off[ia + 1] - off[ia]).astype(np.uint64)
+    nb = (off[ib + 1] - off[ib]).astype(np.uint64)
+    na2, nb2 = na.copy(), nb.copy()
+    na2[0] -= 1  # every set holds 8 or more: the pair's probes drop by one
+    nb2[0] -= 1
+    for hna, hnb, ok in ((na2, nb2, False), (na, nb, True)):
+        _native.check(lib.setops_hip_batch_rebind(gb.handle, gb.dA_beg.data_ptr(), gb.dA_end.data_ptr(),
+                                                  gb.dB_beg.data_ptr(), gb.dB_end.data_ptr(), P, hna.ctypes.data,
+                                                  hnb.ctypes.data))
+        gb.run()
+        if ok:
+            gb.check()
+            g_off, g_cnt, g_data = gb.results()
+            for p in range(P):
+                exp = oracle.branchless_merge(sets[ia[p]], sets[ib[p]])
+                assert np.array_equal(g_data[g_off[p]:g_off[p + 1]], exp), p
+        else:
+            with pytest.raises(_native.SetopsError) as ei:
+                gb.check()
+            assert ei.value.code == _native.SETOPS_ERR_INVALID and "differ" in str(ei.value)
+    gb.close()
+
+
 @pytest.mark.parametrize("name", BSR)
 def test_gather_batch_bsr(name):
     from simd_set_operations_amd.batch import GatherBatch
