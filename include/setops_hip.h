@@ -223,7 +223,8 @@ int setops_hip_batch_reserve(setops_hip_batch *handle, uint64_t npairs, uint64_t
 /* One request-loop step on a gather handle, enqueued on `stream` with one call (nothing waits):
  * copies the packed step arrays h_idx -> d_idx (uint64: a_beg[P], a_end[P], b_beg[P], b_end[P],
  * then, with nfurther >= 0, x_off[P+1], x_beg[nfurther], x_end[nfurther]; h_idx in pinned host
- * memory), rebinds the handle to them (h_na / h_nb: the pairs' sizes, as for
+ * memory; h_idx NULL: no copy, d_idx already holds them, e.g. host memory mapped for the device),
+ * rebinds the handle to them (h_na / h_nb: the pairs' sizes, as for
  * setops_hip_batch_rebind), attaches the further sets (nfurther < 0: none), runs it with outputs
  * (as setops_hip_batch_run), then enqueues the copies of d_out_off[0 .. P] to h_out_off (none if
  * h_out_off is NULL, e.g. for offsets in mapped host memory) and of d_out[0 .. copy_words) to
@@ -234,16 +235,14 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
                           uint64_t out_cap, uint64_t *d_out_off, uint32_t *d_counts, unsigned long long *d_totals,
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream);
 
-/* One step slot of the native request loop: a gather handle over the corpus pool, the staging of
- * its steps' packed rows (h_idx: idx_words uint64 words of pinned host memory, d_idx as many on the
- * device), its per-pair counts (d_counts, max_pairs words) and totals (d_totals, 2 words), and the
- * result words a step may hold (out_words).  A step's offsets and results are written by its
- * kernels straight into host memory that the loop maps for the device (one buffer per slot,
- * allocated when the loop starts): there are no copies back. */
+/* One step slot of the native request loop: a gather handle over the corpus pool, the words of
+ * packed rows a step may hold (idx_words, setops_hip_batch_step's layout), its per-pair counts
+ * (d_counts, max_pairs words) and totals (d_totals, 2 words), and the result words a step may hold
+ * (out_words).  A step's rows are read by its kernels, and its offsets and results written by
+ * them, in host memory that the loop maps for the device (buffers per slot, allocated when the
+ * loop starts): there is no upload and no copy back. */
 typedef struct setops_hip_serve_slot {
     setops_hip_batch *handle;
-    uint64_t *h_idx;
-    uint64_t *d_idx;
     uint64_t idx_words;
     uint64_t out_words;
     uint32_t *d_counts;

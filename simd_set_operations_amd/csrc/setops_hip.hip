@@ -1463,14 +1463,14 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream)
 {
     if (!handle) return fail(SETOPS_ERR_INVALID, "handle is NULL");
-    if (!h_idx || !d_idx || !d_out_off || (copy_words && (!h_out || !d_out)))
+    if (!d_idx || !d_out_off || (copy_words && (!h_out || !d_out)))
         return fail(SETOPS_ERR_INVALID, "NULL argument");
     if (copy_words > out_cap) return fail(SETOPS_ERR_INVALID, "copy_words exceeds out_cap");
     if (npairs == 0) return fail(SETOPS_ERR_INVALID, "batch has no pairs");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint64_t P = npairs;
     const uint64_t words = 4 * P + (nfurther >= 0 ? P + 1 + 2 * (uint64_t)nfurther : 0);
-    HIP_TRY(hipMemcpyAsync(d_idx, h_idx, words * 8, hipMemcpyHostToDevice, s));
+    if (h_idx) HIP_TRY(hipMemcpyAsync(d_idx, h_idx, words * 8, hipMemcpyHostToDevice, s));
     int rc = setops_hip_batch_rebind(handle, d_idx, d_idx + P, d_idx + 2 * P, d_idx + 3 * P, P, h_na, h_nb);
     if (rc != SETOPS_OK) return rc;
     if (nfurther >= 0) {
@@ -1514,14 +1514,16 @@ struct ServeState {
     void *host = nullptr;
     uint64_t *h_off = nullptr, *d_off = nullptr;
     uint32_t *h_res = nullptr, *d_res = nullptr;
+    // the step's packed rows, read by the kernels from mapped host memory too (no upload)
+    void *rows_host = nullptr;
+    uint64_t *h_rows = nullptr, *d_rows = nullptr;
 };
 
-// Copies a step's packed rows for queries q0 .. q1-1 into the slot's pinned staging.
-void pack_step(const setops_hip_serve_slot &sl, const setops_hip_serve_queries *qs, uint64_t q0, uint64_t q1)
+// Writes a step's packed rows for queries q0 .. q1-1 to h (the slot's mapped rows).
+void pack_step(uint64_t *h, const setops_hip_serve_queries *qs, uint64_t q0, uint64_t q1)
 {
     const uint64_t P = q1 - q0, nq = qs->nq;
     const int64_t x0 = qs->xoff[q0], X = qs->xoff[q1] - x0;
-    uint64_t *h = sl.h_idx;
     for (int r = 0; r < 4; r++) std::memcpy(h + r * P, qs->pairs + r * nq + q0, P * 8);
     for (uint64_t j = 0; j <= P; j++) h[4 * P + j] = (uint64_t)(qs->xoff[q0 + j] - x0);
     std::memcpy(h + 5 * P + 1, qs->xb + x0, (size_t)X * 8);
@@ -1544,7 +1546,7 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
     uint64_t idx_min = ~0ull, out_min = ~0ull;
     for (int i = 0; i < nslots; i++) {
         const setops_hip_serve_slot &sl = slots[i];
-        if (!sl.handle || !sl.h_idx || !sl.d_idx || !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
+        if (!sl.handle || !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
             return fail(SETOPS_ERR_INVALID, "slot %d: NULL buffer or no pair slots", i);
         idx_min = std::min(idx_min, sl.idx_words);
         out_min = std::min(out_min, sl.out_words);
@@ -1569,6 +1571,7 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
             if (st[i].q1 > st[i].q0) (void)hipEventSynchronize(st[i].done);
             if (st[i].done) (void)hipEventDestroy(st[i].done);
             if (st[i].host) (void)hipHostFree(st[i].host);
+            if (st[i].rows_host) (void)hipHostFree(st[i].rows_host);
         }
         return rc;
     };
@@ -1592,6 +1595,16 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
         s.d_off = static_cast<uint64_t *>(dev);
         s.h_res = reinterpret_cast<uint32_t *>(static_cast<char *>(s.host) + off_bytes);
         s.d_res = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + off_bytes);
+        if (hipHostMalloc(&s.rows_host, (size_t)slots[i].idx_words * 8, hipHostMallocMapped) != hipSuccess) {
+            (void)hipGetLastError();
+            s.rows_host = nullptr;
+            return finish(fail(SETOPS_ERR_NOMEM, "hipHostMalloc of the step rows failed"));
+        }
+        void *rdev = nullptr;
+        if (hipHostGetDevicePointer(&rdev, s.rows_host, 0) != hipSuccess || !rdev)
+            return finish(fail(SETOPS_ERR_HIP, "hipHostGetDevicePointer failed"));
+        s.h_rows = static_cast<uint64_t *>(s.rows_host);
+        s.d_rows = static_cast<uint64_t *>(rdev);
     }
 
     using clock = std::chrono::steady_clock;
@@ -1622,10 +1635,10 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
                 e++;
             }
             if (e > head) {
-                pack_step(sl, qs, head, e);
+                pack_step(ss.h_rows, qs, head, e);
                 const uint64_t P = e - head;
                 const int64_t X = qs->xoff[e] - qs->xoff[head];
-                int rc = setops_hip_batch_step(sl.handle, sl.h_idx, sl.d_idx, P, X, qs->na + head, qs->nb + head,
+                int rc = setops_hip_batch_step(sl.handle, nullptr, ss.d_rows, P, X, qs->na + head, qs->nb + head,
                                                ss.d_res, sl.out_words, ss.d_off, sl.d_counts, sl.d_totals,
                                                nullptr, nullptr, 0, sl.stream);
                 if (rc != SETOPS_OK) return finish(rc);
