@@ -236,16 +236,14 @@ int setops_hip_batch_step(setops_hip_batch *handle, const uint64_t *h_idx, uint6
                           uint64_t *h_out_off, uint32_t *h_out, uint64_t copy_words, void *stream);
 
 /* One step slot of the native request loop: a gather handle over the corpus pool, the words of
- * packed rows a step may hold (idx_words, setops_hip_batch_step's layout), its per-pair counts
- * (d_counts, max_pairs words) and totals (d_totals, 2 words), and the result words a step may hold
- * (out_words).  A step's rows are read by its kernels, and its offsets and results written by
+ * packed rows a step may hold (idx_words, setops_hip_batch_step's layout), its totals (d_totals,
+ * 2 words on the device), and the result words a step may hold (out_words).  A step's rows are read by its kernels, and its offsets and results written by
  * them, in host memory that the loop maps for the device (buffers per slot, allocated when the
  * loop starts): there is no upload and no copy back. */
 typedef struct setops_hip_serve_slot {
     setops_hip_batch *handle;
     uint64_t idx_words;
     uint64_t out_words;
-    uint32_t *d_counts;
     unsigned long long *d_totals;
     uint64_t max_pairs;
     void *stream;

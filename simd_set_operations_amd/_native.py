@@ -54,8 +54,7 @@ class ServeSlot(ctypes.Structure):
     """setops_hip_serve_slot: a step slot of the native request loop."""
 
     _fields_ = [("handle", ctypes.c_void_p), ("idx_words", ctypes.c_uint64), ("out_words", ctypes.c_uint64),
-                ("d_counts", ctypes.c_void_p), ("d_totals", ctypes.c_void_p), ("max_pairs", ctypes.c_uint64),
-                ("stream", ctypes.c_void_p)]
+                ("d_totals", ctypes.c_void_p), ("max_pairs", ctypes.c_uint64), ("stream", ctypes.c_void_p)]
 
 
 class ServeQueries(ctypes.Structure):

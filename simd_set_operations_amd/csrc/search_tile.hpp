@@ -434,40 +434,68 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
         const uint32_t t = tile0 + blockIdx.x;
         if (t >= plan.ntiles) return;
         // ---- 1. segment table, probe windows -------------------------------------------------------
-        // DIRECT (the launch for plan.direct: one pair, or a batch whose pairs all lie in one tile):
-        // nothing planned the tile; each pair is one whole segment
+        // DIRECT (the launch for plan.direct: one pair, or a batch of at most NT pairs): nothing
+        // planned the tiles.  A batch's pair p owns the units (probes and padding) from the sum over
+        // the pairs before it (a block scan, one pair per thread), and a tile's pairs are those its
+        // unit range meets.  plan.single is uniform: every thread takes the branch or none.
         static_assert(!DIRECT || !BSR, "a direct tile holds plain pairs");
         constexpr bool direct = DIRECT;
-        if (direct && tid == 0 && out.error) *out.error = 0;  // before thread 0's own flags below
-        const uint32_t p_first = direct ? 0u : plan.tile_first[t];
-        const uint32_t p_last = direct ? (plan.single ? 0u : (uint32_t)(in.P - 1)) : plan.tile_last[t];
+        const bool dbatch = direct && !plan.single;
+        if (direct && t == 0 && tid == 0 && out.error) *out.error = 0;  // before the flags below
         const uint64_t u0 = (uint64_t)t * U;
         uint32_t ulen = (uint32_t)((u0 + U < plan.utotal) ? U : plan.utotal - u0);
+        uint32_t d_first = 0, d_last = 0;
+        if constexpr (DIRECT) {
+            if (!plan.single) {
+                // pair tid's first unit and units go to LDS (the probe-key space, staged later)
+                // for the segment table; the tile's first and last pairs after them
+                if (tid == 0) {
+                    sm.pkey[2 * NT] = ~0u;
+                    sm.pkey[2 * NT + 1] = ~0u;
+                }
+                const bool own = tid < in.P;
+                const uint32_t v = own ? (uint32_t)pair_units(in, plan.policy, plan.c0, tid) : 0u;
+                uint32_t tu = 0;
+                const uint32_t pu = block_exclusive_scan<NT>(v, sm.scan_scratch, &tu);
+                // the device sizes must give the units the host planned for (as the plan checks)
+                if (t == 0 && tid == 0 && out.error && (uint64_t)tu != plan.utotal) atomicOr(out.error, 4u);
+                if (own) {
+                    sm.pkey[2 * tid] = pu;
+                    sm.pkey[2 * tid + 1] = v;
+                    const uint64_t ue = u0 + ulen - 1;  // the tile's last unit
+                    if (pu <= u0 && u0 < (uint64_t)pu + v) sm.pkey[2 * NT] = tid;
+                    if (pu <= ue && ue < (uint64_t)pu + v) sm.pkey[2 * NT + 1] = tid;
+                }
+                __syncthreads();  // also: the scratch is scanned again below
+                d_first = sm.pkey[2 * NT];
+                d_last = sm.pkey[2 * NT + 1];
+            }
+        }
+        const uint32_t p_first = direct ? d_first : plan.tile_first[t];
+        const uint32_t p_last = direct ? d_last : plan.tile_last[t];
         uint32_t nseg = p_last - p_first + 1;
         bool bad = p_last < p_first || nseg > (uint32_t)MAXSEG || p_last >= in.P;
         if (bad) nseg = 0;
         if (t == 0 && tid == 0 && out.error && !direct && plan.uoff[in.P] != plan.utotal) atomicOr(out.error, 4u);
-        // a direct batch: pair p's first unit is the sum of the units (probes and padding) of the
-        // pairs before it (a block scan; plan.single is uniform, so every thread takes it or none)
-        uint32_t du = 0;
-        if constexpr (DIRECT) {
-            if (!plan.single) {
-                const uint32_t v = (tid < nseg) ? (uint32_t)pair_units(in, plan.policy, plan.c0, tid) : 0u;
-                uint32_t tu = 0;
-                du = block_exclusive_scan<NT>(v, sm.scan_scratch, &tu);
-                // the device sizes must give the units the host planned for (as the plan checks)
-                if (tid == 0 && out.error && (uint64_t)tu != plan.utotal) atomicOr(out.error, 4u);
-                __syncthreads();  // the scratch is scanned again below
-            }
+        if (dbatch && tid == 0 && !bad) {  // the tile maps the placement reads
+            plan.tile_first[t] = p_first;
+            plan.tile_last[t] = p_last;
         }
+        uint32_t seg_pu = 0;  // direct batch: the segment's pair's first unit
         uint32_t cp = 0;
         uint64_t gp = 0;
         uint32_t swap = 0;
         // the span of the tile's larger B sets, for the top level of the B-pool index (1b)
         unsigned long long gmin = ~0ull, gmax = 0;
         if (tid < nseg) {
-            const SegWin w = direct ? (plan.single ? direct_window<true>(plan) : search_pair_window(in, tid, du))
-                                    : seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            SegWin w;
+            if (dbatch) {
+                const uint32_t p = p_first + tid;
+                seg_pu = sm.pkey[2 * p];
+                w = search_direct_window(in, p, seg_pu, sm.pkey[2 * p + 1], u0, ulen);
+            } else {
+                w = direct ? direct_window<true>(plan) : seg_window(in, plan, t, p_first, nseg, tid, ulen);
+            }
             swap = (w.nb < w.na) ? 1u : 0u;  // probes come from the smaller side
             gp = (swap ? w.b : w.a) + w.i0;
             const uint32_t n = (uint32_t)(w.i1 - w.i0);
@@ -875,7 +903,10 @@ __global__ __launch_bounds__(NT, 4) void search_tile_kernel(PairBatch in, TilePl
             const uint32_t c = before(sm.seg_u[tid + 1]) - c0v;
             const uint32_t f = sm.seg_flags[tid];
             const uint64_t p = (uint64_t)p_first + tid;
-            if (f & 1u) plan.pair_local[p] = c0v;
+            if (f & 1u) {
+                plan.pair_local[p] = c0v;
+                if (dbatch) plan.uoff[p] = seg_pu;  // for the placement (no plan wrote it)
+            }
             if (plan.one_tile && out.out_off) out.out_off[p] = c0v;  // every pair lies in the one tile
             if (out.counts) {
                 if ((f & 3u) == 3u) out.counts[p] = c;

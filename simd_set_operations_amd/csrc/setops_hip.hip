@@ -523,8 +523,8 @@ void launch_search_t(setops_hip_batch *h, const TileOut &out, hipStream_t stream
 {
     if constexpr (!BSR) {
         if (h->plan.direct) {
-            hipLaunchKernelGGL((search_tile_kernel<kNT, KS, kMaxSeg, kCapB, BSR, CO, SMODE, true>), dim3(1), dim3(kNT),
-                               0, stream, h->in, h->plan, out, 0u);
+            hipLaunchKernelGGL((search_tile_kernel<kNT, KS, kMaxSeg, kCapB, BSR, CO, SMODE, true>),
+                               dim3((uint32_t)h->plan.ntiles), dim3(kNT), 0, stream, h->in, h->plan, out, 0u);
             return;
         }
     }
@@ -706,15 +706,18 @@ int batch_launch(setops_hip_batch *h, uint32_t *d_out, uint32_t *d_out_s, uint64
     out.error = h->d_error;
     // a run on another stream than the last one would share the workspace and staging with it
     if (h->ran && h->last_stream != stream) HIP_TRY(hipStreamSynchronize(h->last_stream));
-    // one tile of a plain search-policy batch with room for every probe's result: the tile's slot
-    // is the output and the tile kernel writes the offsets and totals (no scan, no placement); its
-    // direct form takes the pairs as they are (no plan launch either)
-    if (!h->plan.single)
-        h->plan.direct = (!count_only && h->plan.ntiles == 1 && !h->bsr && h->plan.policy == POLICY_SEARCH &&
-                          h->P <= (uint64_t)kMaxSeg && cap >= h->plan.utotal - h->P * h->plan.c0)
-                             ? 1u
-                             : 0u;
-    h->plan.one_tile = h->plan.direct;
+    // a plain search-policy batch of at most kNT pairs runs the search tile's direct form, which
+    // finds its pairs itself (no plan launch): in one tile with room for every probe's result, the
+    // tile's slot is the output and the tile writes the offsets and totals (no scan, no
+    // placement); over a few tiles (a fused placement follows), when no per-pair counts are asked
+    // for (they would be added up across tiles from zero, and nothing zeroes them)
+    if (!h->plan.single) {
+        const bool small = !count_only && !h->bsr && h->plan.policy == POLICY_SEARCH && h->P <= (uint64_t)kNT;
+        const bool one = h->plan.ntiles == 1 && h->P <= (uint64_t)kMaxSeg && cap >= h->plan.utotal - h->P * h->plan.c0;
+        const bool few = h->plan.ntiles > 1 && h->plan.ntiles <= kFuseScanTiles && d_counts == nullptr;
+        h->plan.direct = (small && (one || few)) ? 1u : 0u;
+    }
+    h->plan.one_tile = (h->plan.direct && h->plan.ntiles == 1) ? 1u : 0u;
     if (!count_only) {
         int rc = stage_reserve(h);
         if (rc != SETOPS_OK) return rc;
@@ -1546,7 +1549,7 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
     uint64_t idx_min = ~0ull, out_min = ~0ull;
     for (int i = 0; i < nslots; i++) {
         const setops_hip_serve_slot &sl = slots[i];
-        if (!sl.handle || !sl.d_counts || !sl.d_totals || sl.max_pairs == 0)
+        if (!sl.handle || !sl.d_totals || sl.max_pairs == 0)
             return fail(SETOPS_ERR_INVALID, "slot %d: NULL buffer or no pair slots", i);
         idx_min = std::min(idx_min, sl.idx_words);
         out_min = std::min(out_min, sl.out_words);
@@ -1639,7 +1642,7 @@ int setops_hip_serve(const setops_hip_serve_slot *slots, int nslots, const setop
                 const uint64_t P = e - head;
                 const int64_t X = qs->xoff[e] - qs->xoff[head];
                 int rc = setops_hip_batch_step(sl.handle, nullptr, ss.d_rows, P, X, qs->na + head, qs->nb + head,
-                                               ss.d_res, sl.out_words, ss.d_off, sl.d_counts, sl.d_totals,
+                                               ss.d_res, sl.out_words, ss.d_off, nullptr, sl.d_totals,
                                                nullptr, nullptr, 0, sl.stream);
                 if (rc != SETOPS_OK) return finish(rc);
                 hipStream_t s = static_cast<hipStream_t>(sl.stream);

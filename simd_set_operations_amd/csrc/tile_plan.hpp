@@ -785,9 +785,11 @@ __device__ __forceinline__ SegWin direct_window(const TilePlan &plan)
     return w;
 }
 
-// Pair p as one whole segment of a direct search batch (all its pairs in one tile, nothing
-// planned), starting at tile unit u: probes from the smaller side, all of them in this tile.
-__device__ __forceinline__ SegWin search_pair_window(const PairBatch &in, uint32_t p, uint32_t u)
+// Pair p of a direct search batch (nothing planned) as a segment of the tile of units
+// [u0, u0 + ulen): the pair owns units [pu, pu + units), its probes first (from the smaller side),
+// and meets the tile.  What seg_window gives for it from a plan.
+__device__ __forceinline__ SegWin search_direct_window(const PairBatch &in, uint32_t p, uint64_t pu, uint64_t units,
+                                                       uint64_t u0, uint32_t ulen)
 {
     SegWin w;
     w.p = p;
@@ -795,12 +797,16 @@ __device__ __forceinline__ SegWin search_pair_window(const PairBatch &in, uint32
     w.b = in.b_beg[p];
     w.na = in.a_end[p] - w.a;
     w.nb = in.b_end[p] - w.b;
-    w.i0 = 0;
-    w.i1 = w.na < w.nb ? w.na : w.nb;
+    const uint64_t np = w.na < w.nb ? w.na : w.nb;
+    const uint64_t t1 = u0 + ulen;
+    w.i0 = pu < u0 ? ((u0 - pu) < np ? (u0 - pu) : np) : 0;
+    uint64_t i1 = t1 > pu ? t1 - pu : 0;
+    if (i1 > np) i1 = np;
+    w.i1 = i1 < w.i0 ? w.i0 : i1;
     w.j0 = 0;
     w.j1 = 0;
-    w.u = u;
-    w.flags = 3u;
+    w.u = pu > u0 ? (uint32_t)(pu - u0) : 0u;
+    w.flags = (pu >= u0 ? 1u : 0u) | (pu + units <= t1 ? 2u : 0u);
     return w;
 }
 

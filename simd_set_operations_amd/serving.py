@@ -320,7 +320,7 @@ class Engine:
         # rows and results: as many words per step as the Python loop's buffers hold
         return _native.ServeSlot(
             sl.handle.value, min(sl.h_idx.numel(), sl.d_idx.numel()), min(sl.out.numel(), sl.h_res.numel()),
-            sl.d_cnt.data_ptr(), sl.d_tot.data_ptr(), self.max_pairs, sl.stream.cuda_stream)
+            sl.d_tot.data_ptr(), self.max_pairs, sl.stream.cuda_stream)
 
     def _serve_native(self, ids, ks, arrivals, keep_results, timeout_s):
         import gc

@@ -132,7 +132,7 @@ def test_serve_loop_argument_errors_before_device_use():
     """The native request loop checks its slots and its query stream before any device work."""
     lib = _native.hip()
     fake = 16  # never dereferenced: every case below fails validation first
-    sl = _native.ServeSlot(fake, 64, 128, fake, fake, 4, None)
+    sl = _native.ServeSlot(fake, 64, 128, fake, 4, None)
     slots = (_native.ServeSlot * 2)(sl, sl)
     pairs = np.zeros(4, np.uint64)
     na = np.ones(1, np.uint64)
@@ -157,7 +157,7 @@ def test_serve_loop_argument_errors_before_device_use():
     need[1] = 1 << 20  # a result bound larger than the slots' result buffers
     assert serve(2) == _native.SETOPS_ERR_CAPACITY
     need[1] = 8
-    bad = _native.ServeSlot(None, 64, 128, fake, fake, 4, None)
+    bad = _native.ServeSlot(None, 64, 128, fake, 4, None)
     assert serve(2, slots_=(_native.ServeSlot * 2)(sl, bad)) == _native.SETOPS_ERR_INVALID
     arr2 = np.array([1.0, 0.5])  # arrivals must ascend
     pairs2, na2, xoff2 = np.zeros(8, np.uint64), np.ones(2, np.uint64), np.zeros(3, np.int64)
