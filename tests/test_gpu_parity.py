@@ -681,6 +681,17 @@ def test_served_queries_alone_and_in_the_mix():
     for q in range(ids.shape[0]):
         assert np.array_equal(r3[q], res[q]), q
     eng3.close()
+    # an admission budget that holds a few queries at a time: every query still answered, in more
+    # steps (the budget, not the pair slots, stops each admission)
+    _, cost = serving.query_cost_words(corpus, ids)
+    budget_words = 4 * int(cost.max())
+    announce(f"serving budget {budget_words} words")
+    engb = serving.Engine(corpus, "hip_galloping", max_pairs=256, budget_bytes=4 * budget_words)
+    _, rb, sb = engb.serve(ids, ks, np.zeros(ids.shape[0]), keep_results=True)
+    assert sb["served"] == ids.shape[0] and sb["steps"] > sa["steps"]
+    for q in range(ids.shape[0]):
+        assert np.array_equal(rb[q], res[q]), q
+    engb.close()
 
 
 def test_batch_graph_replays_match():
